@@ -1,0 +1,62 @@
+// alg_desc.hpp -- host-side description of one generated fast algorithm and
+// the argument blocks of the stage kernels.
+//
+// Every device address in a plan is written as (role, offset): role selects
+// one of four base pointers that are only known at call time (the caller's
+// A, B, C and the workspace arena), so a plan built once for a shape can be
+// replayed for any buffers (and captured in a graph) without re-uploading
+// descriptors.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+enum FmRole : int32_t { FM_ROLE_A = 0, FM_ROLE_B = 1, FM_ROLE_C = 2, FM_ROLE_WS = 3 };
+
+struct FmBases {
+  void* p[4];
+};
+
+// A column-major operand: element (i, j) at base[role] + off + i + j * ld.
+struct FmOp {
+  int32_t role;
+  int32_t pad;
+  int64_t off;  // in elements
+  int64_t ld;   // in elements
+};
+
+// S/T stage: for every node q, read the node's operand (ops[q]) split in
+// bm x bn blocks of rows x cols, write NOUT outputs of rows x cols at
+// ws + out_off + (q * NOUT + o) * out_slab with leading dimension out_ld.
+// W stage: read R products of node q from the same slab layout and write the
+// m x n blocks of ops[q] (C_ij = sum W M (+ beta C_ij)).
+struct FmStageArgs {
+  FmBases bases;
+  const FmOp* ops;
+  int32_t rows, cols, nodes, vec;
+  int64_t out_off, out_ld, out_slab;
+  double x;
+  double beta;
+};
+
+typedef hipError_t (*FmStageLaunch)(const FmStageArgs& a, hipStream_t s);
+
+struct FmAlgDesc {
+  const char* name;
+  int m, k, n, R;
+  int ns, nt;  // materialised S / T outputs per node
+  bool apa;
+  // per product r: kind 1 = materialised output index, 0 = block index
+  const int* a_kind;
+  const int* a_index;
+  double (*a_coef)(int r, double x);
+  const int* b_kind;
+  const int* b_index;
+  double (*b_coef)(int r, double x);
+  FmStageLaunch s_d, t_d, w_d, s_f, t_f, w_f;
+};
+
+// generated registry (csrc/gen/registry.cpp): one accessor per algorithm
+typedef const FmAlgDesc* (*FmAlgAccessor)();
+extern const FmAlgAccessor fm_all_algorithms[];
+extern const int fm_num_all_algorithms;

@@ -1,0 +1,205 @@
+// gemm_mfma.hip -- grouped leaf GEMM on CDNA4 matrix cores (FP64 and FP32).
+//
+// One launch runs every product of a recursion level (the BFS form of the
+// reference's leaf dgemm calls, linalg/linalg.hpp:261-269 reached from
+// algorithms/strassen.hpp:385-388) plus any peeling products
+// (linalg/linalg.hpp:213-243).  Work-group w takes output tile w of the
+// concatenated tile list; products are found by division (uniform shapes) or
+// by binary search over tile_begin.
+//
+// Tile: BM x BN per 256-thread work group, 2 x 2 waves, each wave a
+// (BM/2) x (BN/2) block of 16 x 16 MFMA tiles, K staged through LDS in BK=16
+// slices, double-buffered with a register prefetch of the next slice.
+// v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32 are fed with the B tile as
+// the instruction's first operand and the A tile as the second, so the
+// instruction's output row index runs along C's columns and its lane index
+// along C's rows: each store instruction writes 16 consecutive rows of 4
+// columns, i.e. contiguous 128-byte runs of column-major C.
+#include <hip/hip_runtime.h>
+
+#include "gemm_mfma.hpp"
+
+template <typename T>
+struct FmMfma;
+
+template <>
+struct FmMfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  __device__ __forceinline__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+
+template <>
+struct FmMfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  __device__ __forceinline__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// Map a work-group id to a tile id so that each of the 8 XCDs (which receive
+// work groups round-robin) walks a contiguous range of tiles: neighbouring
+// tiles share A/B panels in that XCD's L2.
+__device__ __forceinline__ int fm_xcd_remap(int wg, int nwg) {
+  const int full = nwg & ~7;
+  if (wg < full) return (wg & 7) * (full >> 3) + (wg >> 3);
+  return wg;
+}
+
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) {
+  using Op = FmMfma<T>;
+  using acc_t = typename Op::acc_t;
+  constexpr int NT = 64 * WM * WN;
+  constexpr int WTM = BM / WM, WTN = BN / WN;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  constexpr int BKP = BK + 1;
+  constexpr int A_PER = BM * BK / NT, B_PER = BK * BN / NT;
+  constexpr int A_KSTEP = NT / BM;  // k columns covered by one pass of the A tile
+  constexpr int B_NSTEP = NT / BK;  // n columns covered by one pass of the B tile
+  static_assert(NT % BM == 0 && NT % BK == 0, "tile/thread mismatch");
+  static_assert(A_PER * NT == BM * BK && B_PER * NT == BK * BN, "tile/thread mismatch");
+  static_assert(TM >= 1 && TN >= 1 && BK % 4 == 0, "bad tile");
+
+  __shared__ T As[2][BK][BM];
+  __shared__ T Bs[2][BN][BKP];
+
+  const int wg = fm_xcd_remap(blockIdx.x, gridDim.x);
+  int g;
+  if (args.uniform) {
+    const int per = args.descs[0].tiles_m * args.descs[0].tiles_n;
+    g = wg / per;
+  } else {
+    int lo = 0, hi = args.ndesc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (args.descs[mid].tile_begin <= wg)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    g = lo;
+  }
+  const FmGemmDesc d = args.descs[g];
+  const int local = wg - d.tile_begin;
+  const int tmi = local % d.tiles_m, tni = local / d.tiles_m;
+  const int m0 = tmi * BM, n0 = tni * BN;
+  const int M = d.M, N = d.N, K = d.K;
+  const T* __restrict__ A = reinterpret_cast<const T*>(args.bases.p[d.a.role]) + d.a.off;
+  const T* __restrict__ B = reinterpret_cast<const T*>(args.bases.p[d.b.role]) + d.b.off;
+  T* __restrict__ C = reinterpret_cast<T*>(args.bases.p[d.c.role]) + d.c.off;
+  const int64_t lda = d.a.ld, ldb = d.b.ld, ldc = d.c.ld;
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+
+  // per-thread load coordinates (fixed over the K loop)
+  const int a_row = tid % BM, a_k = tid / BM;
+  const int b_k = tid % BK, b_col = tid / BK;
+  const bool a_row_ok = (m0 + a_row) < M;
+  const T* a_ptr = A + (m0 + a_row) + (int64_t)a_k * lda;
+  const T* b_ptr = B + b_k + (int64_t)(n0 + b_col) * ldb;
+
+  acc_t acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  T ra[A_PER], rb[B_PER];
+
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) {
+      const int gk = k0 + a_k + i * A_KSTEP;
+      ra[i] = (a_row_ok && gk < K) ? a_ptr[(int64_t)(k0 + i * A_KSTEP) * lda] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) {
+      const int gk = k0 + b_k;
+      const int gn = n0 + b_col + i * B_NSTEP;
+      rb[i] = (gk < K && gn < N) ? b_ptr[k0 + (int64_t)(i * B_NSTEP) * ldb] : T(0);
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int i = 0; i < A_PER; ++i) As[buf][a_k + i * A_KSTEP][a_row] = ra[i];
+#pragma unroll
+    for (int i = 0; i < B_PER; ++i) Bs[buf][b_col + i * B_NSTEP][b_k] = rb[i];
+  };
+
+  const int nk = (K + BK - 1) / BK;
+  gload(0);
+  sstore(0);
+  __syncthreads();
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) gload((kt + 1) * BK);
+#pragma unroll
+    for (int k4 = 0; k4 < BK / 4; ++k4) {
+      const int kidx = k4 * 4 + (lane >> 4);
+      T af[TM], bf[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) af[i] = As[cur][kidx][wm * WTM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bf[j] = Bs[cur][wn * WTN + j * 16 + (lane & 15)][kidx];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = Op::mma(bf[j], af[i], acc[i][j]);
+    }
+    if (kt + 1 < nk) sstore(cur ^ 1);
+    __syncthreads();
+  }
+
+  const T alpha = (T)d.alpha, beta = (T)d.beta;
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int row = m0 + wm * WTM + i * 16 + (lane & 15);
+    if (row >= M) continue;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = n0 + wn * WTN + j * 16 + 4 * (lane >> 4) + v;
+        if (col < N) {
+          T* p = C + row + (int64_t)col * ldc;
+          T r = alpha * acc[i][j][v];
+          if (beta != T(0)) r += beta * *p;
+          *p = r;
+        }
+      }
+    }
+  }
+}
+
+template <typename T>
+static hipError_t fm_gemm_launch(int cfg, const FmGemmArgs& a, hipStream_t s) {
+  if (a.total_tiles <= 0) return hipSuccess;
+  const dim3 grid(a.total_tiles), block(256);
+  switch (cfg) {
+    case FM_TILE_128x128:
+      hipLaunchKernelGGL((fm_gemm_kernel<T, 128, 128, 16, 2, 2>), grid, block, 0, s, a);
+      break;
+    case FM_TILE_64x64:
+      hipLaunchKernelGGL((fm_gemm_kernel<T, 64, 64, 16, 2, 2>), grid, block, 0, s, a);
+      break;
+    case FM_TILE_128x64:
+      hipLaunchKernelGGL((fm_gemm_kernel<T, 128, 64, 16, 2, 2>), grid, block, 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((fm_gemm_kernel<T, 64, 128, 16, 2, 2>), grid, block, 0, s, a);
+      break;
+  }
+  return hipGetLastError();
+}
+
+hipError_t fm_gemm_launch_f64(int cfg, const FmGemmArgs& a, hipStream_t s) {
+  return fm_gemm_launch<double>(cfg, a, s);
+}
+
+hipError_t fm_gemm_launch_f32(int cfg, const FmGemmArgs& a, hipStream_t s) {
+  return fm_gemm_launch<float>(cfg, a, s);
+}

@@ -1,0 +1,230 @@
+// stage_kernels.hpp -- HBM-bound linear-combination stages of one recursion
+// level, shared by every generated algorithm (csrc/gen/alg_*.hip).
+//
+// One launch covers all nodes of a level (grid.y = node): the BFS form of the
+// reference's per-product S_Add/T_Add/M_Add calls (algorithms/strassen.hpp:9-376).
+// Each lane owns VEC consecutive rows of one column of a block position,
+// loads that position from every block of the node's operand once, and the
+// generated combine function produces every output from registers, so each
+// source element is read from HBM once per level and each output written once.
+//
+// Arithmetic goes through fm_add/fm_sub/fm_mul/fm_neg, which never contract
+// into FMA: the combination of terms is rounded exactly like the reference's
+// separate multiply/add expressions (codegen/gen.py:216-233).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "alg_desc.hpp"
+
+__device__ __forceinline__ double fm_add(double a, double b) { return __dadd_rn(a, b); }
+__device__ __forceinline__ double fm_sub(double a, double b) { return __dadd_rn(a, -b); }
+__device__ __forceinline__ double fm_mul(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ double fm_neg(double a) { return -a; }
+__device__ __forceinline__ float fm_add(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ float fm_sub(float a, float b) { return __fadd_rn(a, -b); }
+__device__ __forceinline__ float fm_mul(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ float fm_neg(float a) { return -a; }
+
+template <typename T, int VEC>
+struct FmVec {
+  T v[VEC];
+};
+
+template <typename T, int VEC>
+__device__ __forceinline__ FmVec<T, VEC> fm_load(const T* p) {
+  FmVec<T, VEC> r;
+  if constexpr (VEC == 1) {
+    r.v[0] = *p;
+  } else {
+    typedef T vt __attribute__((ext_vector_type(VEC)));
+    vt x = *reinterpret_cast<const vt*>(p);
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) r.v[i] = x[i];
+  }
+  return r;
+}
+
+template <typename T, int VEC>
+__device__ __forceinline__ void fm_store(T* p, const FmVec<T, VEC>& r) {
+  if constexpr (VEC == 1) {
+    *p = r.v[0];
+  } else {
+    typedef T vt __attribute__((ext_vector_type(VEC)));
+    vt x;
+#pragma unroll
+    for (int i = 0; i < VEC; ++i) x[i] = r.v[i];
+    *reinterpret_cast<vt*>(p) = x;
+  }
+}
+
+constexpr int FM_STAGE_THREADS = 256;
+
+// SIDE 0: A operand (m x k blocks, U/SX, outputs S); SIDE 1: B (k x n, V/TX, T).
+template <class Cmb, typename T, int VEC, int SIDE>
+__global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage(FmStageArgs a) {
+  constexpr int BR = SIDE == 0 ? Cmb::m : Cmb::k;
+  constexpr int BC = SIDE == 0 ? Cmb::k : Cmb::n;
+  constexpr int NB = BR * BC;
+  constexpr int NOUT = SIDE == 0 ? Cmb::NS : Cmb::NT;
+  constexpr int NOUT_ARR = NOUT > 0 ? NOUT : 1;
+  const int q = blockIdx.y;
+  const FmOp op = a.ops[q];
+  const T* src = reinterpret_cast<const T*>(a.bases.p[op.role]) + op.off;
+  T* dst = reinterpret_cast<T*>(a.bases.p[FM_ROLE_WS]) + a.out_off +
+           (int64_t)q * NOUT * a.out_slab;
+  const uint32_t rv = (uint32_t)a.rows / VEC;
+  const uint32_t total = rv * (uint32_t)a.cols;
+  const int64_t ld = op.ld;
+  for (uint32_t e = blockIdx.x * FM_STAGE_THREADS + threadIdx.x; e < total;
+       e += gridDim.x * FM_STAGE_THREADS) {
+    const uint32_t j = e / rv;
+    const uint32_t i = (e - j * rv) * VEC;
+    FmVec<T, VEC> v[NB];
+#pragma unroll
+    for (int p = 0; p < NB; ++p) {
+      const int bi = p / BC, bj = p % BC;
+      v[p] = fm_load<T, VEC>(src + (int64_t)bi * a.rows + i + ((int64_t)bj * a.cols + j) * ld);
+    }
+    FmVec<T, VEC> o[NOUT_ARR];
+#pragma unroll
+    for (int l = 0; l < VEC; ++l) {
+      T in[NB];
+#pragma unroll
+      for (int p = 0; p < NB; ++p) in[p] = v[p].v[l];
+      T out[NOUT_ARR];
+      if constexpr (SIDE == 0)
+        Cmb::s_combine(in, out, a.x);
+      else
+        Cmb::t_combine(in, out, a.x);
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t) o[t].v[l] = out[t];
+    }
+#pragma unroll
+    for (int t = 0; t < NOUT; ++t)
+      fm_store<T, VEC>(dst + (int64_t)t * a.out_slab + i + (int64_t)j * a.out_ld, o[t]);
+  }
+}
+
+// C_ij (+)= sum_q W[ij, q] M_q for every node; M_q of node q at
+// ws + out_off + (q * R + r) * out_slab.
+template <class Cmb, typename T, int VEC, bool BETA>
+__global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage(FmStageArgs a) {
+  constexpr int R = Cmb::R;
+  constexpr int MN = Cmb::m * Cmb::n;
+  const int q = blockIdx.y;
+  const FmOp op = a.ops[q];
+  T* c = reinterpret_cast<T*>(a.bases.p[op.role]) + op.off;
+  const T* mb = reinterpret_cast<const T*>(a.bases.p[FM_ROLE_WS]) + a.out_off +
+                (int64_t)q * R * a.out_slab;
+  const uint32_t rv = (uint32_t)a.rows / VEC;
+  const uint32_t total = rv * (uint32_t)a.cols;
+  const int64_t ld = op.ld;
+  const T beta = (T)a.beta;
+  for (uint32_t e = blockIdx.x * FM_STAGE_THREADS + threadIdx.x; e < total;
+       e += gridDim.x * FM_STAGE_THREADS) {
+    const uint32_t j = e / rv;
+    const uint32_t i = (e - j * rv) * VEC;
+    FmVec<T, VEC> mv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      mv[r] = fm_load<T, VEC>(mb + (int64_t)r * a.out_slab + i + (int64_t)j * a.out_ld);
+    FmVec<T, VEC> o[MN];
+#pragma unroll
+    for (int l = 0; l < VEC; ++l) {
+      T in[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) in[r] = mv[r].v[l];
+      T out[MN];
+      Cmb::w_combine(in, out, a.x);
+#pragma unroll
+      for (int t = 0; t < MN; ++t) o[t].v[l] = out[t];
+    }
+#pragma unroll
+    for (int t = 0; t < MN; ++t) {
+      const int bi = t / Cmb::n, bj = t % Cmb::n;
+      T* p = c + (int64_t)bi * a.rows + i + ((int64_t)bj * a.cols + j) * ld;
+      if constexpr (BETA) {
+        FmVec<T, VEC> old = fm_load<T, VEC>(p);
+#pragma unroll
+        for (int l = 0; l < VEC; ++l) o[t].v[l] = fm_add(o[t].v[l], fm_mul(beta, old.v[l]));
+      }
+      fm_store<T, VEC>(p, o[t]);
+    }
+  }
+}
+
+inline dim3 fm_stage_grid(const FmStageArgs& a, int vec) {
+  const uint64_t total = (uint64_t)(a.rows / vec) * (uint64_t)a.cols;
+  uint64_t gx = (total + FM_STAGE_THREADS - 1) / FM_STAGE_THREADS;
+  // enough work groups to fill 256 CUs several times over, across all nodes
+  uint64_t cap = 8192 / (uint64_t)(a.nodes > 0 ? a.nodes : 1);
+  if (cap < 4) cap = 4;
+  if (gx > cap) gx = cap;
+  if (gx < 1) gx = 1;
+  return dim3((unsigned)gx, (unsigned)a.nodes, 1);
+}
+
+template <class Cmb, typename T, int SIDE>
+hipError_t fm_launch_in_stage(const FmStageArgs& a, hipStream_t s) {
+  constexpr int VMAX = sizeof(T) == 8 ? 2 : 4;
+  if (a.vec == VMAX) {
+    hipLaunchKernelGGL((fm_in_stage<Cmb, T, VMAX, SIDE>), fm_stage_grid(a, VMAX),
+                       dim3(FM_STAGE_THREADS), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((fm_in_stage<Cmb, T, 1, SIDE>), fm_stage_grid(a, 1),
+                       dim3(FM_STAGE_THREADS), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+template <class Cmb, typename T>
+hipError_t fm_launch_w_stage(const FmStageArgs& a, hipStream_t s) {
+  constexpr int VMAX = sizeof(T) == 8 ? 2 : 4;
+  const bool beta = a.beta != 0.0;
+  if (a.vec == VMAX) {
+    if (beta)
+      hipLaunchKernelGGL((fm_w_stage<Cmb, T, VMAX, true>), fm_stage_grid(a, VMAX),
+                         dim3(FM_STAGE_THREADS), 0, s, a);
+    else
+      hipLaunchKernelGGL((fm_w_stage<Cmb, T, VMAX, false>), fm_stage_grid(a, VMAX),
+                         dim3(FM_STAGE_THREADS), 0, s, a);
+  } else {
+    if (beta)
+      hipLaunchKernelGGL((fm_w_stage<Cmb, T, 1, true>), fm_stage_grid(a, 1),
+                         dim3(FM_STAGE_THREADS), 0, s, a);
+    else
+      hipLaunchKernelGGL((fm_w_stage<Cmb, T, 1, false>), fm_stage_grid(a, 1),
+                         dim3(FM_STAGE_THREADS), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+#define FM_DEFINE_ALGORITHM(NAME, GNS, APA)                                                   \
+  static hipError_t fm_##NAME##_s_d(const FmStageArgs& a, hipStream_t s) {                   \
+    return fm_launch_in_stage<GNS::Combine, double, 0>(a, s);                                 \
+  }                                                                                          \
+  static hipError_t fm_##NAME##_t_d(const FmStageArgs& a, hipStream_t s) {                   \
+    return fm_launch_in_stage<GNS::Combine, double, 1>(a, s);                                 \
+  }                                                                                          \
+  static hipError_t fm_##NAME##_w_d(const FmStageArgs& a, hipStream_t s) {                   \
+    return fm_launch_w_stage<GNS::Combine, double>(a, s);                                     \
+  }                                                                                          \
+  static hipError_t fm_##NAME##_s_f(const FmStageArgs& a, hipStream_t s) {                   \
+    return fm_launch_in_stage<GNS::Combine, float, 0>(a, s);                                  \
+  }                                                                                          \
+  static hipError_t fm_##NAME##_t_f(const FmStageArgs& a, hipStream_t s) {                   \
+    return fm_launch_in_stage<GNS::Combine, float, 1>(a, s);                                  \
+  }                                                                                          \
+  static hipError_t fm_##NAME##_w_f(const FmStageArgs& a, hipStream_t s) {                   \
+    return fm_launch_w_stage<GNS::Combine, float>(a, s);                                      \
+  }                                                                                          \
+  const FmAlgDesc* fm_alg_##NAME() {                                                        \
+    static const FmAlgDesc d = {                                                             \
+        #NAME, GNS::Combine::m, GNS::Combine::k, GNS::Combine::n, GNS::Combine::R,           \
+        GNS::Combine::NS, GNS::Combine::NT, APA, GNS::a_kind, GNS::a_index, GNS::a_coef,     \
+        GNS::b_kind, GNS::b_index, GNS::b_coef, fm_##NAME##_s_d, fm_##NAME##_t_d,           \
+        fm_##NAME##_w_d, fm_##NAME##_s_f, fm_##NAME##_t_f, fm_##NAME##_w_f};                  \
+    return &d;                                                                               \
+  }

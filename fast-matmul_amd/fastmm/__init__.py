@@ -1,0 +1,262 @@
+"""fastmm -- fast (Strassen-like) matrix multiplication on one MI355X.
+
+Host-side mirror of the reference's API over the C ABI of libfastmm.so:
+
+* ``Matrix``: the column-major view type of linalg/linalg.hpp:41-192
+  (``m``, ``n``, ``stride``, ``multiplier``, ``Subblock``, ``Submatrix``),
+  backed by a GPU tensor;
+* one namespace object per algorithm with ``FastMatmul(A, B, C, num_steps,
+  x=1e-8, alpha=1, beta=0) -> ms`` (algorithms/strassen.hpp:662-722), reached
+  as ``fastmm.strassen``, ``fastmm.grey424_26_206`` ... or ``fastmm.alg(name)``;
+* ``RunAlgorithm(ENUM_NAME, A, B, C, num_steps)`` and ``MatMul(A, B, C, beta)``
+  (algorithms/all_algorithms.hpp:93-243, linalg/linalg.hpp:261-269);
+* tensor-level ``fastmatmul`` / ``gemm`` for callers that hold torch tensors.
+
+All arithmetic runs in the HIP kernels of libfastmm.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional
+
+import torch
+
+from . import _lib
+from . import tables
+from ._lib import LIB, check
+
+__all__ = ["Matrix", "fastmatmul", "gemm", "MatMul", "RunAlgorithm", "alg", "algorithm_names",
+           "workspace_bytes", "release_caches"]
+
+_DT = {torch.float64: _lib.FM_F64, torch.float32: _lib.FM_F32}
+
+
+def algorithm_names():
+    return [LIB.fm_algorithm_name(i).decode() for i in range(LIB.fm_num_algorithms())]
+
+
+def _col_major(t: torch.Tensor, name: str) -> torch.Tensor:
+    if t.dim() != 2:
+        raise ValueError("%s must be 2-D" % name)
+    if t.stride(0) == 1 and t.stride(1) >= max(1, t.shape[0]):
+        return t
+    if t.shape[1] == 1 and t.stride(0) == 1:
+        return t
+    raise ValueError("%s must be a column-major view (stride(0) == 1); use "
+                     "fastmm.col_major(%s)" % (name, name))
+
+
+def col_major(t: torch.Tensor) -> torch.Tensor:
+    """Column-major copy of a 2-D tensor (no copy if it already is one)."""
+    if t.stride(0) == 1 and t.stride(1) >= max(1, t.shape[0]):
+        return t
+    return t.t().contiguous().t()
+
+
+def empty(m: int, n: int, dtype=torch.float64, device="cuda") -> torch.Tensor:
+    """Uninitialised column-major m x n tensor."""
+    return torch.empty((n, m), dtype=dtype, device=device).t()
+
+
+def _ld(t: torch.Tensor) -> int:
+    return max(1, t.stride(1) if t.shape[1] > 1 else max(t.shape[0], 1))
+
+
+def _stream_handle(t: torch.Tensor):
+    return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def fastmatmul(alg_name: str, A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None,
+               num_steps: int = 1, x: float = 1e-8, alpha: float = 1.0, beta: float = 0.0,
+               mode: str = "plain", workspace: Optional[torch.Tensor] = None,
+               timed: bool = False):
+    """C := alpha * A @ B + beta * C with ``num_steps`` levels of ``alg_name``.
+
+    A, B, C are column-major GPU tensors of one dtype (float64 or float32).
+    Returns C, or (C, ms) when ``timed``."""
+    A = _col_major(A, "A")
+    B = _col_major(B, "B")
+    m, k = A.shape
+    k2, n = B.shape
+    if k2 != k:
+        raise ValueError("inner dimensions differ: %d vs %d" % (k, k2))
+    if A.dtype != B.dtype or A.dtype not in _DT:
+        raise TypeError("A and B must share a float64/float32 dtype")
+    if not A.is_cuda or not B.is_cuda:
+        raise ValueError("A and B must be GPU tensors")
+    if C is None:
+        C = empty(m, n, A.dtype, A.device)
+        if beta != 0.0:
+            raise ValueError("beta != 0 needs an input C")
+    C = _col_major(C, "C")
+    if C.shape != (m, n) or C.dtype != A.dtype:
+        raise ValueError("C must be %d x %d of dtype %s" % (m, n, A.dtype))
+    ws_ptr, ws_bytes = None, 0
+    if workspace is not None:
+        ws_ptr, ws_bytes = ctypes.c_void_p(workspace.data_ptr()), workspace.numel() * workspace.element_size()
+    ms = ctypes.c_float(0.0)
+    st = LIB.fm_fastmatmul(alg_name.encode(), _DT[A.dtype], _lib.MODES[mode], m, k, n,
+                           ctypes.c_void_p(A.data_ptr()), _ld(A), ctypes.c_void_p(B.data_ptr()),
+                           _ld(B), ctypes.c_void_p(C.data_ptr()), _ld(C), int(num_steps),
+                           float(x), float(alpha), float(beta), _stream_handle(A), ws_ptr,
+                           ws_bytes, ctypes.byref(ms) if timed else None)
+    check(st)
+    return (C, ms.value) if timed else C
+
+
+def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alpha: float = 1.0,
+         beta: float = 0.0) -> torch.Tensor:
+    """Classical C := alpha * A @ B + beta * C on the library's MFMA kernel."""
+    A = _col_major(A, "A")
+    B = _col_major(B, "B")
+    m, k = A.shape
+    n = B.shape[1]
+    if C is None:
+        C = empty(m, n, A.dtype, A.device)
+    C = _col_major(C, "C")
+    st = LIB.fm_gemm(_DT[A.dtype], m, k, n, ctypes.c_void_p(A.data_ptr()), _ld(A),
+                     ctypes.c_void_p(B.data_ptr()), _ld(B), ctypes.c_void_p(C.data_ptr()), _ld(C),
+                     float(alpha), float(beta), _stream_handle(A))
+    check(st)
+    return C
+
+
+def workspace_bytes(alg_name: str, m: int, k: int, n: int, num_steps: int,
+                    dtype=torch.float64, mode: str = "plain") -> int:
+    out = ctypes.c_size_t(0)
+    check(LIB.fm_workspace_query(alg_name.encode(), _DT[dtype], _lib.MODES[mode], m, k, n,
+                                 num_steps, ctypes.byref(out)))
+    return out.value
+
+
+def release_caches():
+    LIB.fm_release_caches()
+
+
+def profile_enable(on: bool = True):
+    """Bracket every library launch with HIP events on its stream."""
+    LIB.fm_profile_enable(1 if on else 0)
+
+
+def profile_read():
+    """{stage: (ms_total, launches)} accumulated since the last read."""
+    p = _lib.FmProfile()
+    check(LIB.fm_profile_read(ctypes.byref(p)))
+    return {nm: (p.ms[i], p.launches[i]) for i, nm in enumerate(_lib.STAGES)}
+
+
+# ---------------------------------------------------------------------------
+# Reference-shaped API
+# ---------------------------------------------------------------------------
+class Matrix:
+    """Column-major view with a stride and a scalar multiplier
+    (linalg/linalg.hpp:41-192), stored in a GPU tensor.
+
+    ``Matrix(m, n)`` allocates (uninitialised, like ``new Scalar[m*n]``);
+    ``Matrix(tensor)`` wraps a column-major 2-D tensor without copying."""
+
+    def __init__(self, m, n: Optional[int] = None, multiplier: float = 1.0,
+                 dtype=torch.float64, device="cuda"):
+        if isinstance(m, torch.Tensor):
+            self.data = _col_major(m, "data")
+        else:
+            n = m if n is None else n
+            self.data = empty(m, n, dtype, device)
+        self._multiplier = multiplier
+
+    @classmethod
+    def from_numpy(cls, a, device="cuda"):
+        t = torch.as_tensor(a).to(device)
+        return cls(col_major(t))
+
+    def numpy(self):
+        return self.data.cpu().numpy()
+
+    def m(self):
+        return self.data.shape[0]
+
+    def n(self):
+        return self.data.shape[1]
+
+    def stride(self):
+        return _ld(self.data)
+
+    def multiplier(self):
+        return self._multiplier
+
+    def set_multiplier(self, v):
+        self._multiplier = v
+
+    def UpdateMultiplier(self, v):
+        self._multiplier *= v
+
+    def __call__(self, i, j):
+        return self.data[i, j].item()
+
+    def Submatrix(self, start_row, start_col, num_rows, num_cols):
+        return Matrix(self.data[start_row:start_row + num_rows, start_col:start_col + num_cols],
+                      multiplier=self._multiplier)
+
+    def Subblock(self, num_block_rows, num_block_cols, row_ind, col_ind):
+        """Block (row_ind, col_ind), 1-indexed, of floor size
+        (linalg/linalg.hpp:97-102, 179-184)."""
+        rs = self.m() // num_block_rows
+        cs = self.n() // num_block_cols
+        return self.Submatrix(rs * (row_ind - 1), cs * (col_ind - 1), rs, cs)
+
+
+class _Algorithm:
+    """The namespace of one generated algorithm (e.g. ``strassen``)."""
+
+    def __init__(self, name):
+        self.name = name
+        m, k, n, r = (ctypes.c_int() for _ in range(4))
+        check(LIB.fm_algorithm_info(name.encode(), ctypes.byref(m), ctypes.byref(k),
+                                    ctypes.byref(n), ctypes.byref(r)))
+        self.dims = (m.value, k.value, n.value)
+        self.rank = r.value
+
+    def FastMatmul(self, A: Matrix, B: Matrix, C: Matrix, num_steps: int, x: float = 1e-8,
+                   alpha: float = 1.0, beta: float = 0.0) -> float:
+        """C := alpha A B + beta C; returns elapsed device ms.  The leaf
+        multiplier is alpha * B.multiplier * C.multiplier as in
+        algorithms/strassen.hpp:669,380-383; no Matrix is modified."""
+        eff = alpha * B.multiplier() * C.multiplier()
+        _, ms = fastmatmul(self.name, A.data, B.data, C.data, num_steps, x, eff, beta, timed=True)
+        return ms
+
+    def __repr__(self):
+        return "<fastmm algorithm %s <%d,%d,%d;%d>>" % ((self.name,) + self.dims + (self.rank,))
+
+
+_algs = {}
+
+
+def alg(name: str) -> _Algorithm:
+    if name not in _algs:
+        _algs[name] = _Algorithm(name)
+    return _algs[name]
+
+
+def MatMul(A: Matrix, B: Matrix, C: Matrix, beta: float = 0.0):
+    """C <- C.multiplier * A B + beta C (linalg/linalg.hpp:261-269)."""
+    gemm(A.data, B.data, C.data, C.multiplier(), beta)
+
+
+def RunAlgorithm(algorithm: str, A: Matrix, B: Matrix, C: Matrix, num_steps: int) -> float:
+    """Dispatch by the reference's enum name (algorithms/all_algorithms.hpp:93-243)."""
+    if algorithm not in tables.ENUM_TO_NAMESPACE:
+        raise KeyError("unknown algorithm %r" % algorithm)
+    ns = tables.ENUM_TO_NAMESPACE[algorithm]
+    if algorithm == "MKL":
+        return alg(ns).FastMatmul(A, B, C, 0)
+    return alg(ns).FastMatmul(A, B, C, num_steps, tables.default_x(algorithm, num_steps))
+
+
+def __getattr__(name):
+    if name.startswith("_"):
+        raise AttributeError(name)
+    try:
+        return alg(name)
+    except KeyError:
+        raise AttributeError(name) from None

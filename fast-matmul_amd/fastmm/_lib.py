@@ -1,0 +1,92 @@
+"""ctypes binding of libfastmm.so (the C ABI declared in include/fastmm.h).
+
+The library is built in-tree by fast-matmul_amd/build.py.  There is no
+fallback: if the library is missing, importing the package raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfastmm.so")
+
+FM_OK = 0
+FM_ERR_INVALID_ARG = 1
+FM_ERR_UNKNOWN_ALG = 2
+FM_ERR_SHAPE = 3
+FM_ERR_WORKSPACE = 4
+FM_ERR_HIP = 5
+FM_ERR_UNSUPPORTED = 6
+
+FM_F64 = 0
+FM_F32 = 1
+
+MODES = {"plain": 0, "split_bf16": 1, "ozaki_int8": 2}
+
+# every symbol include/fastmm.h declares
+EXPORTS = (
+    "fm_fastmatmul", "fm_workspace_query", "fm_gemm", "fm_num_algorithms",
+    "fm_algorithm_name", "fm_algorithm_info", "fm_status_string", "fm_last_error",
+    "fm_release_caches", "fm_profile_enable", "fm_profile_read",
+)
+
+STAGES = ("S", "T", "W", "LEAF", "PEEL")
+
+
+class FmProfile(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * 5), ("launches", ctypes.c_int64 * 5)]
+
+
+class FastMMError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(msg)
+        self.status = status
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            "libfastmm.so is not built (expected at %s); run "
+            "`python fast-matmul_amd/build.py` or __graft_entry__.build()" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    i64, vp, c_int, c_double = ctypes.c_int64, ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    lib.fm_fastmatmul.restype = c_int
+    lib.fm_fastmatmul.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int,
+                                  vp, i64, vp, i64, vp, i64, c_int, c_double, c_double,
+                                  c_double, vp, vp, ctypes.c_size_t,
+                                  ctypes.POINTER(ctypes.c_float)]
+    lib.fm_workspace_query.restype = c_int
+    lib.fm_workspace_query.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int,
+                                       c_int, ctypes.POINTER(ctypes.c_size_t)]
+    lib.fm_gemm.restype = c_int
+    lib.fm_gemm.argtypes = [c_int, c_int, c_int, c_int, vp, i64, vp, i64, vp, i64, c_double,
+                            c_double, vp]
+    lib.fm_num_algorithms.restype = c_int
+    lib.fm_algorithm_name.restype = ctypes.c_char_p
+    lib.fm_algorithm_name.argtypes = [c_int]
+    lib.fm_algorithm_info.restype = c_int
+    lib.fm_algorithm_info.argtypes = [ctypes.c_char_p] + [ctypes.POINTER(c_int)] * 4
+    lib.fm_status_string.restype = ctypes.c_char_p
+    lib.fm_status_string.argtypes = [c_int]
+    lib.fm_last_error.restype = ctypes.c_char_p
+    lib.fm_release_caches.restype = None
+    lib.fm_profile_enable.restype = None
+    lib.fm_profile_enable.argtypes = [c_int]
+    lib.fm_profile_read.restype = c_int
+    lib.fm_profile_read.argtypes = [ctypes.POINTER(FmProfile)]
+    return lib
+
+
+LIB = _load()
+
+
+def check(status: int):
+    if status == FM_OK:
+        return
+    msg = "%s: %s" % (LIB.fm_status_string(status).decode(), LIB.fm_last_error().decode())
+    if status == FM_ERR_UNKNOWN_ALG:
+        raise KeyError(msg)
+    if status in (FM_ERR_INVALID_ARG, FM_ERR_SHAPE, FM_ERR_UNSUPPORTED):
+        raise ValueError(msg)
+    raise FastMMError(status, msg)

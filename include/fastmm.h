@@ -1,0 +1,123 @@
+/*
+ * fastmm.h -- C ABI of the MI355X fast matrix multiplication engine.
+ *
+ * The reference exposes its hot path as header-only C++ templates, one
+ * namespace per algorithm:
+ *
+ *   template <typename Scalar>
+ *   double <alg>::FastMatmul(Matrix<Scalar>& A, Matrix<Scalar>& B, Matrix<Scalar>& C,
+ *                            int num_steps, double x = 1e-8,
+ *                            Scalar alpha = Scalar(1.0), Scalar beta = Scalar(0.0));
+ *   (algorithms/strassen.hpp:662-722; the same shape in every generated header,
+ *    e.g. algorithms/fast424_26_206.hpp:2947-3007)
+ *
+ * with Matrix<Scalar> a column-major view (linalg/linalg.hpp:41-192) and the
+ * enum dispatcher RunAlgorithm (algorithms/all_algorithms.hpp:93-243).
+ * These entry points replace that path with device pointers, explicit
+ * leading dimensions, a status code and no hidden state:
+ *
+ *   - matrices are column-major; A is m x k, B is k x n, C is m x n;
+ *     lda >= max(1, m), ldb >= max(1, k), ldc >= max(1, m); all pointers are
+ *     device pointers of the current HIP device;
+ *   - C := alpha * A * B + beta * C, computed with num_steps levels of the
+ *     named algorithm, dynamic peeling for non-divisible shapes, exactly the
+ *     reference's recursion (multipliers folded into the leaf products,
+ *     beta applied at the top level only; with num_steps == 0 the reference's
+ *     base case ignores beta and so does this one);
+ *   - unlike the reference, no caller state changes (the reference leaves
+ *     C's multiplier at alpha, algorithms/strassen.hpp:669,380-383).
+ *
+ * A C++ header with the reference's Matrix<Scalar> / <alg>::FastMatmul
+ * spelling over these entry points is include/fastmm/fast_matmul.hpp.
+ */
+#ifndef FASTMM_H_
+#define FASTMM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  FM_OK = 0,
+  FM_ERR_INVALID_ARG = 1,   /* null pointer, negative size, ld too small   */
+  FM_ERR_UNKNOWN_ALG = 2,   /* name not in the registry                    */
+  FM_ERR_SHAPE = 3,         /* num_steps too deep: a block dimension is 0  */
+  FM_ERR_WORKSPACE = 4,     /* caller workspace smaller than required      */
+  FM_ERR_HIP = 5,           /* HIP runtime error (see fm_last_error)       */
+  FM_ERR_UNSUPPORTED = 6    /* dtype/mode combination not available        */
+} fm_status;
+
+typedef enum { FM_F64 = 0, FM_F32 = 1 } fm_dtype;
+
+/* Leaf arithmetic.  PLAIN: leaves in the data type on its MFMA path.
+ * SPLIT_BF16: FP32 data, leaves from split-bfloat16 products with FP32
+ * accumulation.  OZAKI_INT8: FP64 data, leaves from int8 slices with exact
+ * int32 accumulation and FP64 reconstruction. */
+typedef enum { FM_MODE_PLAIN = 0, FM_MODE_SPLIT_BF16 = 1, FM_MODE_OZAKI_INT8 = 2 } fm_mode;
+
+/* <alg>::FastMatmul (algorithms/strassen.hpp:662-722).  `alg` is the
+ * reference's namespace name ("strassen", "grey424_26_206", ...).  `x` is
+ * the APA parameter (ignored by exact algorithms).  `stream` is a
+ * hipStream_t (NULL = default stream).  `workspace` may be NULL (the library
+ * then uses its own cached device arena) or a device buffer of at least
+ * fm_workspace_query() bytes.  If `ms_out` is non-NULL the call waits for
+ * completion and stores the elapsed device time in ms (the reference's
+ * FastMatmul returns its elapsed ms, algorithms/strassen.hpp:708-721). */
+fm_status fm_fastmatmul(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k, int n,
+                        const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                        int64_t ldc, int num_steps, double x, double alpha, double beta,
+                        void* stream, void* workspace, size_t workspace_bytes,
+                        float* ms_out);
+
+/* Workspace bytes fm_fastmatmul needs for this problem (the reference sizes
+ * its BFS buffers in util/MemoryManager.hpp:65-110). */
+fm_status fm_workspace_query(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k,
+                             int n, int num_steps, size_t* bytes);
+
+/* Classical C := alpha * A * B + beta * C on the same MFMA kernels
+ * (MatMul, linalg/linalg.hpp:261-269). */
+fm_status fm_gemm(fm_dtype dtype, int m, int k, int n, const void* A, int64_t lda,
+                  const void* B, int64_t ldb, void* C, int64_t ldc, double alpha,
+                  double beta, void* stream);
+
+/* Registry (algorithms/all_algorithms.hpp:52-90, 247-326). */
+int fm_num_algorithms(void);
+const char* fm_algorithm_name(int index);
+fm_status fm_algorithm_info(const char* alg, int* m, int* k, int* n, int* rank);
+
+const char* fm_status_string(fm_status s);
+/* Text of the last error on the calling thread ("" if none). */
+const char* fm_last_error(void);
+
+/* Per-stage timing.  While enabled, every launch fm_fastmatmul/fm_gemm
+ * issues is bracketed by HIP events on the caller's stream (the stream the
+ * kernels run on); fm_profile_read waits for them, returns the accumulated
+ * device time and launch count per stage kind and resets the counters. */
+typedef enum {
+  FM_STAGE_S = 0,     /* S_r = sum U A      (HBM-bound)                     */
+  FM_STAGE_T = 1,     /* T_r = sum V B      (HBM-bound)                     */
+  FM_STAGE_W = 2,     /* C_ij = sum W M (+ beta C)  (HBM-bound)             */
+  FM_STAGE_LEAF = 3,  /* grouped leaf products (MFMA-bound)                 */
+  FM_STAGE_PEEL = 4,  /* dynamic-peeling products                           */
+  FM_STAGE_COUNT = 5
+} fm_stage_kind;
+
+typedef struct {
+  double ms[FM_STAGE_COUNT];
+  int64_t launches[FM_STAGE_COUNT];
+} fm_profile;
+
+void fm_profile_enable(int on);
+fm_status fm_profile_read(fm_profile* out);
+
+/* Drop cached plans and the library-owned workspace arena. */
+void fm_release_caches(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTMM_H_ */

@@ -1,0 +1,74 @@
+"""The C ABI library (no GPU needed): it loads, exports every entry point
+include/fastmm.h declares, lists the registry, and rejects bad arguments
+before touching a device."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "fastmm.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:const char\*|fm_status|int|void)\s+\**(fm_\w+)\(", txt, re.M)))
+
+
+def test_exports_every_declared_symbol():
+    from fastmm import _lib
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared()
+    assert len(names) >= 9
+    for nm in names:
+        assert hasattr(lib, nm), nm
+    assert sorted(_lib.EXPORTS) == names
+
+
+def test_registry_matches_tables():
+    import fastmm
+    from fastmm import tables
+    assert sorted(fastmm.algorithm_names()) == tables.algorithm_names()
+    for name in tables.algorithm_names():
+        t = tables.get_table(name)
+        a = fastmm.alg(name)
+        assert a.dims == (t.m, t.k, t.n) and a.rank == t.R
+
+
+def test_argument_errors():
+    from fastmm import _lib
+    L = _lib.LIB
+    nul = ctypes.c_void_p(0)
+    # unknown algorithm
+    st = L.fm_fastmatmul(b"nope", 0, 0, 4, 4, 4, nul, 4, nul, 4, nul, 4, 1, 1e-8, 1.0, 0.0,
+                         None, None, 0, None)
+    assert st == _lib.FM_ERR_UNKNOWN_ALG
+    # leading dimension too small
+    st = L.fm_fastmatmul(b"strassen", 0, 0, 8, 8, 8, nul, 4, nul, 8, nul, 8, 1, 1e-8, 1.0, 0.0,
+                         None, None, 0, None)
+    assert st == _lib.FM_ERR_INVALID_ARG
+    # null pointers
+    st = L.fm_fastmatmul(b"strassen", 0, 0, 8, 8, 8, nul, 8, nul, 8, nul, 8, 1, 1e-8, 1.0, 0.0,
+                         None, None, 0, None)
+    assert st == _lib.FM_ERR_INVALID_ARG
+    # negative steps / dims
+    st = L.fm_fastmatmul(b"strassen", 0, 0, -1, 8, 8, nul, 8, nul, 8, nul, 8, 1, 1e-8, 1.0, 0.0,
+                         None, None, 0, None)
+    assert st == _lib.FM_ERR_INVALID_ARG
+    # empty output is a no-op
+    st = L.fm_fastmatmul(b"strassen", 0, 0, 0, 8, 8, nul, 1, nul, 8, nul, 1, 1, 1e-8, 1.0, 0.0,
+                         None, None, 0, None)
+    assert st == _lib.FM_OK
+    assert L.fm_status_string(_lib.FM_ERR_SHAPE) == b"shape not supported at this depth"
+
+
+def test_workspace_query_and_depth_limit():
+    import fastmm
+    # C2-sq: 14.27 GB of S/T/M buffers for 2 BFS levels of <4,2,4;26> at 8192^3
+    b = fastmm.workspace_bytes("grey424_26_206", 8192, 8192, 8192, 2)
+    assert 14e9 < b < 14.5e9
+    assert fastmm.workspace_bytes("strassen", 64, 64, 64, 0) == 0
+    with pytest.raises(ValueError):
+        fastmm.workspace_bytes("strassen", 3, 3, 3, 2)
+    with pytest.raises(KeyError):
+        fastmm.workspace_bytes("nope", 8, 8, 8, 1)
