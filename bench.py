@@ -62,6 +62,9 @@ def main():
     ap.add_argument("--workload", default="c2sq", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the host CPU baseline")
     ap.add_argument("--cpu-n", type=int, default=4096, help="square size of the CPU sample")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one 1 GiB read + 1 GiB write elementwise kernel, the "
+                         "known-size reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the fast multiply on a single MI355X "
@@ -116,6 +119,14 @@ def main():
     achieved = F_leaf / (leaf_avg_ms * 1e-3) / 1e12 if leaf_n else None
     stages = {k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps}
               for k, v in prof.items()}
+
+    if args.copy_probe:
+        x = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
+        y = torch.empty_like(x)
+        sync()
+        torch.mul(x, 1.0, out=y)
+        sync()
+        del x, y
 
     cpu = None
     if not args.no_cpu:

@@ -104,24 +104,40 @@ double round_scalar(double v, fm_dtype dt) { return dt == FM_F32 ? (double)(floa
 
 int elem_size(fm_dtype dt) { return dt == FM_F64 ? 8 : 4; }
 
-// Choose the GEMM tile for `count` products of M x N.
-int pick_tile(int64_t M, int64_t N, int64_t count) {
+int g_tile_override = -1;  // fm_set_gemm_tile
+
+// Choose the GEMM tile for `count` products of M x N: the smallest estimated
+// time over the configurations, where a configuration's throughput per tile
+// (eff, measured on MI355X at 8192^3) and its resident work groups per CU
+// set how many rounds of tiles the launch takes.
+int pick_tile(fm_dtype dt, int64_t M, int64_t N, int64_t count) {
+  if (g_tile_override >= 0) return g_tile_override;
   struct C {
-    int cfg, bm, bn;
+    int cfg;
     double eff;
-  } cands[] = {{FM_TILE_128x128, 128, 128, 1.0},
-               {FM_TILE_128x64, 128, 64, 0.9},
-               {FM_TILE_64x128, 64, 128, 0.9},
-               {FM_TILE_64x64, 64, 64, 0.75}};
+    int per_cu;
+  };
+  static const C f64c[] = {{FM_TILE_128x128_W8, 1.00, 2}, {FM_TILE_256x128_W8, 0.94, 1},
+                           {FM_TILE_128x256_W8, 0.94, 1}, {FM_TILE_64x64, 0.80, 4},
+                           {FM_TILE_128x64, 0.77, 2},     {FM_TILE_64x128, 0.77, 2}};
+  static const C f32c[] = {{FM_TILE_256x256_W16, 1.00, 1}, {FM_TILE_256x128_W8, 0.90, 2},
+                           {FM_TILE_128x256_W8, 0.90, 2},  {FM_TILE_128x128_W8, 0.82, 4},
+                           {FM_TILE_64x128, 0.77, 4},      {FM_TILE_128x64, 0.74, 4},
+                           {FM_TILE_64x64, 0.69, 8}};
+  const C* cands = dt == FM_F64 ? f64c : f32c;
+  const int nc = dt == FM_F64 ? 6 : 7;
   int best = FM_TILE_64x64;
   double best_t = 1e300;
-  for (const C& c : cands) {
-    const int64_t tiles = ((M + c.bm - 1) / c.bm) * ((N + c.bn - 1) / c.bn) * count;
-    const int64_t waves = (tiles + 511) / 512;  // 256 CUs x 2 resident groups
-    const double t = (double)waves * c.bm * c.bn / c.eff;
+  for (int i = 0; i < nc; ++i) {
+    int bm, bn;
+    fm_tile_dims(cands[i].cfg, &bm, &bn);
+    const int64_t tiles = ((M + bm - 1) / bm) * ((N + bn - 1) / bn) * count;
+    const int64_t slots = 256LL * cands[i].per_cu;
+    const double rounds = (double)((tiles + slots - 1) / slots);
+    const double t = rounds * bm * bn * cands[i].per_cu / cands[i].eff;
     if (t < best_t * 0.999) {
       best_t = t;
-      best = c.cfg;
+      best = cands[i].cfg;
     }
   }
   return best;
@@ -303,7 +319,7 @@ fm_status build_plan(Plan& P, const FmAlgDesc* alg, fm_dtype dt, int64_t M, int6
       g.beta = 0.0;
       descs.push_back(g);
     }
-    add_gemm_launch(P, descs, pick_tile(d.m, d.n, (int64_t)descs.size()), true);
+    add_gemm_launch(P, descs, pick_tile(dt, d.m, d.n, (int64_t)descs.size()), true);
   }
   for (int l = L - 1; l >= 0; --l) {
     const Dims c = dims[l + 1];
@@ -334,7 +350,7 @@ fm_status build_gemm_plan(Plan& P, fm_dtype dt, int64_t M, int64_t K, int64_t N,
   g.alpha = round_scalar(alpha, dt);
   g.beta = round_scalar(beta, dt);
   std::vector<FmGemmDesc> descs(1, g);
-  add_gemm_launch(P, descs, pick_tile(M, N, 1), true);
+  add_gemm_launch(P, descs, pick_tile(dt, M, N, 1), true);
   P.ws_bytes = 0;
   return FM_OK;
 }
@@ -630,6 +646,14 @@ const char* fm_status_string(fm_status s) {
 }
 
 const char* fm_last_error(void) { return g_err.c_str(); }
+
+fm_status fm_set_gemm_tile(int cfg) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  if (cfg < -1 || cfg >= FM_TILE_NUM) return set_err(FM_ERR_INVALID_ARG, "unknown tile config");
+  g_tile_override = cfg;
+  g_plans.clear();  // plans embed the tile choice
+  return FM_OK;
+}
 
 void fm_profile_enable(int on) {
   std::lock_guard<std::mutex> lock(g_mu);

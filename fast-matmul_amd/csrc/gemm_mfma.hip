@@ -22,12 +22,15 @@
 template <typename T>
 struct FmMfma;
 
+// out_row(g, v): output row held in accumulator item v by lane group g
+// (g = lane / 16; the output column is lane % 16).
 template <>
 struct FmMfma<double> {
   typedef double acc_t __attribute__((ext_vector_type(4)));
   __device__ __forceinline__ static acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  __device__ __forceinline__ static int out_row(int g, int v) { return g + 4 * v; }
 };
 
 template <>
@@ -36,6 +39,7 @@ struct FmMfma<float> {
   __device__ __forceinline__ static acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  __device__ __forceinline__ static int out_row(int g, int v) { return 4 * g + v; }
 };
 
 // Map a work-group id to a tile id so that each of the 8 XCDs (which receive
@@ -163,7 +167,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
     for (int j = 0; j < TN; ++j) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int col = n0 + wn * WTN + j * 16 + 4 * (lane >> 4) + v;
+        const int col = n0 + wn * WTN + j * 16 + Op::out_row(lane >> 4, v);
         if (col < N) {
           T* p = C + row + (int64_t)col * ldc;
           T r = alpha * acc[i][j][v];
@@ -175,23 +179,28 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
   }
 }
 
+template <typename T, int BM, int BN, int BK, int WM, int WN>
+static void fm_gemm_go(const FmGemmArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((fm_gemm_kernel<T, BM, BN, BK, WM, WN>), dim3(a.total_tiles),
+                     dim3(64 * WM * WN), 0, s, a);
+}
+
 template <typename T>
 static hipError_t fm_gemm_launch(int cfg, const FmGemmArgs& a, hipStream_t s) {
   if (a.total_tiles <= 0) return hipSuccess;
-  const dim3 grid(a.total_tiles), block(256);
   switch (cfg) {
-    case FM_TILE_128x128:
-      hipLaunchKernelGGL((fm_gemm_kernel<T, 128, 128, 16, 2, 2>), grid, block, 0, s, a);
-      break;
-    case FM_TILE_64x64:
-      hipLaunchKernelGGL((fm_gemm_kernel<T, 64, 64, 16, 2, 2>), grid, block, 0, s, a);
-      break;
-    case FM_TILE_128x64:
-      hipLaunchKernelGGL((fm_gemm_kernel<T, 128, 64, 16, 2, 2>), grid, block, 0, s, a);
-      break;
-    default:
-      hipLaunchKernelGGL((fm_gemm_kernel<T, 64, 128, 16, 2, 2>), grid, block, 0, s, a);
-      break;
+    case FM_TILE_128x128: fm_gemm_go<T, 128, 128, 16, 2, 2>(a, s); break;
+    case FM_TILE_64x64: fm_gemm_go<T, 64, 64, 16, 2, 2>(a, s); break;
+    case FM_TILE_128x64: fm_gemm_go<T, 128, 64, 16, 2, 2>(a, s); break;
+    case FM_TILE_64x128: fm_gemm_go<T, 64, 128, 16, 2, 2>(a, s); break;
+    case FM_TILE_128x128_K32: fm_gemm_go<T, 128, 128, 32, 2, 2>(a, s); break;
+    case FM_TILE_128x128_W8: fm_gemm_go<T, 128, 128, 16, 2, 4>(a, s); break;
+    case FM_TILE_128x128_K32_W8: fm_gemm_go<T, 128, 128, 32, 2, 4>(a, s); break;
+    case FM_TILE_256x128_W8: fm_gemm_go<T, 256, 128, 16, 4, 2>(a, s); break;
+    case FM_TILE_128x256_W8: fm_gemm_go<T, 128, 256, 16, 2, 4>(a, s); break;
+    case FM_TILE_256x256_W16: fm_gemm_go<T, 256, 256, 16, 4, 4>(a, s); break;
+    case FM_TILE_64x64_K32: fm_gemm_go<T, 64, 64, 32, 2, 2>(a, s); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }

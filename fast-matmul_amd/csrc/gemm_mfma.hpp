@@ -26,16 +26,30 @@ struct FmGemmArgs {
   int32_t pad;
 };
 
-enum FmTileCfg : int { FM_TILE_128x128 = 0, FM_TILE_64x64 = 1, FM_TILE_128x64 = 2, FM_TILE_64x128 = 3 };
+// Tile configurations of the leaf kernel: BM x BN output tile, BK-deep K
+// slices, WM x WN waves (64 * WM * WN threads).
+enum FmTileCfg : int {
+  FM_TILE_128x128 = 0,         // BK 16, 2x2 waves
+  FM_TILE_64x64 = 1,           // BK 16, 2x2 waves
+  FM_TILE_128x64 = 2,          // BK 16, 2x2 waves
+  FM_TILE_64x128 = 3,          // BK 16, 2x2 waves
+  FM_TILE_128x128_K32 = 4,     // BK 32, 2x2 waves
+  FM_TILE_128x128_W8 = 5,      // BK 16, 2x4 waves
+  FM_TILE_128x128_K32_W8 = 6,  // BK 32, 2x4 waves
+  FM_TILE_256x128_W8 = 7,      // BK 16, 4x2 waves
+  FM_TILE_128x256_W8 = 8,      // BK 16, 2x4 waves
+  FM_TILE_256x256_W16 = 9,     // BK 16, 4x4 waves
+  FM_TILE_64x64_K32 = 10,      // BK 32, 2x2 waves
+  FM_TILE_NUM = 11
+};
 
-// tile sizes of each configuration
+// output tile of each configuration
 inline void fm_tile_dims(int cfg, int* bm, int* bn) {
-  switch (cfg) {
-    case FM_TILE_128x128: *bm = 128; *bn = 128; break;
-    case FM_TILE_64x64: *bm = 64; *bn = 64; break;
-    case FM_TILE_128x64: *bm = 128; *bn = 64; break;
-    default: *bm = 64; *bn = 128; break;
-  }
+  static const int d[FM_TILE_NUM][2] = {{128, 128}, {64, 64}, {128, 64}, {64, 128},
+                                        {128, 128}, {128, 128}, {128, 128}, {256, 128},
+                                        {128, 256}, {256, 256}, {64, 64}};
+  *bm = d[cfg][0];
+  *bn = d[cfg][1];
 }
 
 hipError_t fm_gemm_launch_f64(int cfg, const FmGemmArgs& a, hipStream_t s);

@@ -133,6 +133,11 @@ def release_caches():
     LIB.fm_release_caches()
 
 
+def set_gemm_tile(cfg: int = -1):
+    """Force a leaf-GEMM tile configuration (-1: automatic choice)."""
+    check(LIB.fm_set_gemm_tile(int(cfg)))
+
+
 def profile_enable(on: bool = True):
     """Bracket every library launch with HIP events on its stream."""
     LIB.fm_profile_enable(1 if on else 0)

