@@ -29,6 +29,21 @@ sys.path.insert(0, os.path.join(ROOT, "fast-matmul_amd"))
 
 PEAK_FP64_TF = 78.6
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per leaf launch of the default workload, from the two rocprofv3
+# --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in this file, scaled by a
+# 1 GiB copy kernel measured in the same runs
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_grey424_26_206_fp64_8192.md")
+
+
+def measured_traffic():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch (leaf GEMM):"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 WORKLOADS = {
     # name: (alg, steps, M, K, N)
@@ -170,7 +185,9 @@ def main():
                      "achieved": achieved, "peak": PEAK_FP64_TF, "unit": "TFLOP/s",
                      "frac": (achieved / PEAK_FP64_TF) if achieved else None,
                      "flops_per_launch": F_leaf, "avg_launch_ms": leaf_avg_ms,
-                     "traffic": None},
+                     "traffic": measured_traffic() if args.workload == "c2sq" else None,
+                     "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)
+                     if args.workload == "c2sq" else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

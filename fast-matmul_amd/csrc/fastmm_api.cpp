@@ -159,6 +159,14 @@ void add_gemm_launch(Plan& P, std::vector<FmGemmDesc>& descs, int cfg, bool unif
   L.gm.ndesc = (int32_t)descs.size();
   L.gm.total_tiles = t;
   L.gm.uniform = uniform ? 1 : 0;
+  // 16-byte loads need every A/B column start aligned: even (FP64) or
+  // multiple-of-4 (FP32) offsets and leading dimensions; base pointers are
+  // checked at launch
+  const int vl = P.dtype == FM_F64 ? 2 : 4;
+  bool vec = true;
+  for (const FmGemmDesc& d : descs)
+    vec = vec && d.a.off % vl == 0 && d.a.ld % vl == 0 && d.b.off % vl == 0 && d.b.ld % vl == 0;
+  L.vec_ok = vec ? 1 : 0;
   L.stage = uniform ? FM_STAGE_LEAF : FM_STAGE_PEEL;
   L.cfg = cfg;
   L.count = descs.size();
@@ -411,6 +419,7 @@ fm_status run_plan(const Plan& P, const void* A, const void* B, void* C, void* w
     if (L.kind == LK_GEMM) {
       FmGemmArgs g = L.gm;
       g.bases = bases;
+      g.vec = (L.vec_ok && aligned) ? 1 : 0;
       e = P.dtype == FM_F64 ? fm_gemm_launch_f64(L.cfg, g, s) : fm_gemm_launch_f32(L.cfg, g, s);
     } else {
       FmStageArgs a = L.st;

@@ -7,9 +7,10 @@
 // concatenated tile list; products are found by division (uniform shapes) or
 // by binary search over tile_begin.
 //
-// Tile: BM x BN per 256-thread work group, 2 x 2 waves, each wave a
-// (BM/2) x (BN/2) block of 16 x 16 MFMA tiles, K staged through LDS in BK=16
-// slices, double-buffered with a register prefetch of the next slice.
+// Tile: BM x BN per work group of WM x WN waves, each wave a
+// (BM/WM) x (BN/WN) block of 16 x 16 MFMA tiles, K staged through LDS in
+// BK-deep slices, double-buffered with a register prefetch of the next slice
+// (16-byte loads on interior tiles, predicated element loads on edges).
 // v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32 are fed with the B tile as
 // the instruction's first operand and the A tile as the second, so the
 // instruction's output row index runs along C's columns and its lane index
@@ -59,15 +60,24 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
   constexpr int WTM = BM / WM, WTN = BN / WN;
   constexpr int TM = WTM / 16, TN = WTN / 16;
   constexpr int BKP = BK + 1;
+  // scalar (edge) load mapping
   constexpr int A_PER = BM * BK / NT, B_PER = BK * BN / NT;
   constexpr int A_KSTEP = NT / BM;  // k columns covered by one pass of the A tile
   constexpr int B_NSTEP = NT / BK;  // n columns covered by one pass of the B tile
+  // 16-byte (interior) load mapping: VL consecutive elements per load
+  constexpr int VL = 16 / sizeof(T);
+  constexpr bool CAN_VEC = A_PER % VL == 0 && B_PER % VL == 0 && BM % VL == 0 &&
+                           BK % VL == 0 && (NT * VL) % BM == 0 && (NT * VL) % BK == 0;
+  constexpr int AV_PER = CAN_VEC ? A_PER / VL : 1, BV_PER = CAN_VEC ? B_PER / VL : 1;
+  constexpr int AV_KSTEP = CAN_VEC ? NT * VL / BM : 1;
+  constexpr int BV_NSTEP = CAN_VEC ? NT * VL / BK : 1;
   static_assert(NT % BM == 0 && NT % BK == 0, "tile/thread mismatch");
   static_assert(A_PER * NT == BM * BK && B_PER * NT == BK * BN, "tile/thread mismatch");
   static_assert(TM >= 1 && TN >= 1 && BK % 4 == 0, "bad tile");
+  typedef T vec_t __attribute__((ext_vector_type(VL)));
 
-  __shared__ T As[2][BK][BM];
-  __shared__ T Bs[2][BN][BKP];
+  __shared__ __align__(16) T As[2][BK][BM];
+  __shared__ __align__(16) T Bs[2][BN][BKP];
 
   const int wg = fm_xcd_remap(blockIdx.x, gridDim.x);
   int g;
@@ -99,12 +109,22 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
 
-  // per-thread load coordinates (fixed over the K loop)
+  // Interior tiles of operands with 16-byte aligned columns take unpredicated
+  // 16-byte loads for every full K slice; everything else takes the
+  // predicated element loads.
+  const bool interior = CAN_VEC && args.vec && (m0 + BM <= M) && (n0 + BN <= N);
+
+  // element-load coordinates
   const int a_row = tid % BM, a_k = tid / BM;
   const int b_k = tid % BK, b_col = tid / BK;
   const bool a_row_ok = (m0 + a_row) < M;
   const T* a_ptr = A + (m0 + a_row) + (int64_t)a_k * lda;
   const T* b_ptr = B + b_k + (int64_t)(n0 + b_col) * ldb;
+  // vector-load coordinates
+  const int av_row = (tid % (BM / VL)) * VL, av_k = tid / (BM / VL);
+  const int bv_k = (tid % (BK / VL)) * VL, bv_col = tid / (BK / VL);
+  const T* av_ptr = A + (m0 + av_row) + (int64_t)av_k * lda;
+  const T* bv_ptr = B + bv_k + (int64_t)(n0 + bv_col) * ldb;
 
   acc_t acc[TM][TN];
 #pragma unroll
@@ -114,6 +134,35 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
 
   T ra[A_PER], rb[B_PER];
 
+  auto gload_vec = [&](int k0) {
+    if constexpr (!CAN_VEC) return;
+#pragma unroll
+    for (int i = 0; i < AV_PER; ++i) {
+      const vec_t v = *reinterpret_cast<const vec_t*>(av_ptr + (int64_t)(k0 + i * AV_KSTEP) * lda);
+#pragma unroll
+      for (int e = 0; e < VL; ++e) ra[i * VL + e] = v[e];
+    }
+#pragma unroll
+    for (int i = 0; i < BV_PER; ++i) {
+      const vec_t v = *reinterpret_cast<const vec_t*>(bv_ptr + k0 + (int64_t)(i * BV_NSTEP) * ldb);
+#pragma unroll
+      for (int e = 0; e < VL; ++e) rb[i * VL + e] = v[e];
+    }
+  };
+  auto sstore_vec = [&](int buf) {
+    if constexpr (!CAN_VEC) return;
+#pragma unroll
+    for (int i = 0; i < AV_PER; ++i) {
+      vec_t v;
+#pragma unroll
+      for (int e = 0; e < VL; ++e) v[e] = ra[i * VL + e];
+      *reinterpret_cast<vec_t*>(&As[buf][av_k + i * AV_KSTEP][av_row]) = v;
+    }
+#pragma unroll
+    for (int i = 0; i < BV_PER; ++i)
+#pragma unroll
+      for (int e = 0; e < VL; ++e) Bs[buf][bv_col + i * BV_NSTEP][bv_k + e] = rb[i * VL + e];
+  };
   auto gload = [&](int k0) {
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
@@ -135,12 +184,25 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
   };
 
   const int nk = (K + BK - 1) / BK;
-  gload(0);
-  sstore(0);
+  bool vec_stage = interior && BK <= K;
+  if (vec_stage) {
+    gload_vec(0);
+    sstore_vec(0);
+  } else {
+    gload(0);
+    sstore(0);
+  }
   __syncthreads();
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
-    if (kt + 1 < nk) gload((kt + 1) * BK);
+    const bool more = kt + 1 < nk;
+    vec_stage = interior && (kt + 2) * BK <= K;
+    if (more) {
+      if (vec_stage)
+        gload_vec((kt + 1) * BK);
+      else
+        gload((kt + 1) * BK);
+    }
 #pragma unroll
     for (int k4 = 0; k4 < BK / 4; ++k4) {
       const int kidx = k4 * 4 + (lane >> 4);
@@ -154,7 +216,12 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = Op::mma(bf[j], af[i], acc[i][j]);
     }
-    if (kt + 1 < nk) sstore(cur ^ 1);
+    if (more) {
+      if (vec_stage)
+        sstore_vec(cur ^ 1);
+      else
+        sstore(cur ^ 1);
+    }
     __syncthreads();
   }
 
@@ -200,6 +267,9 @@ static hipError_t fm_gemm_launch(int cfg, const FmGemmArgs& a, hipStream_t s) {
     case FM_TILE_128x256_W8: fm_gemm_go<T, 128, 256, 16, 2, 4>(a, s); break;
     case FM_TILE_256x256_W16: fm_gemm_go<T, 256, 256, 16, 4, 4>(a, s); break;
     case FM_TILE_64x64_K32: fm_gemm_go<T, 64, 64, 32, 2, 2>(a, s); break;
+    case FM_TILE_128x128_W16: fm_gemm_go<T, 128, 128, 16, 4, 4>(a, s); break;
+    case FM_TILE_128x64_W8: fm_gemm_go<T, 128, 64, 16, 2, 4>(a, s); break;
+    case FM_TILE_64x128_W8: fm_gemm_go<T, 64, 128, 16, 4, 2>(a, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

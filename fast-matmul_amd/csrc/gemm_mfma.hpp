@@ -23,7 +23,7 @@ struct FmGemmArgs {
   int32_t ndesc;
   int32_t total_tiles;
   int32_t uniform;  // all products have one shape: tile -> product by division
-  int32_t pad;
+  int32_t vec;      // every A/B column start is 16-byte aligned: vector loads allowed
 };
 
 // Tile configurations of the leaf kernel: BM x BN output tile, BK-deep K
@@ -40,14 +40,18 @@ enum FmTileCfg : int {
   FM_TILE_128x256_W8 = 8,      // BK 16, 2x4 waves
   FM_TILE_256x256_W16 = 9,     // BK 16, 4x4 waves
   FM_TILE_64x64_K32 = 10,      // BK 32, 2x2 waves
-  FM_TILE_NUM = 11
+  FM_TILE_128x128_W16 = 11,    // BK 16, 4x4 waves
+  FM_TILE_128x64_W8 = 12,      // BK 16, 2x4 waves
+  FM_TILE_64x128_W8 = 13,      // BK 16, 4x2 waves
+  FM_TILE_NUM = 14
 };
 
 // output tile of each configuration
 inline void fm_tile_dims(int cfg, int* bm, int* bn) {
   static const int d[FM_TILE_NUM][2] = {{128, 128}, {64, 64}, {128, 64}, {64, 128},
                                         {128, 128}, {128, 128}, {128, 128}, {256, 128},
-                                        {128, 256}, {256, 256}, {64, 64}};
+                                        {128, 256}, {256, 256}, {64, 64},  {128, 128},
+                                        {128, 64},  {64, 128}};
   *bm = d[cfg][0];
   *bn = d[cfg][1];
 }

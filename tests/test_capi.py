@@ -72,3 +72,22 @@ def test_workspace_query_and_depth_limit():
         fastmm.workspace_bytes("strassen", 3, 3, 3, 2)
     with pytest.raises(KeyError):
         fastmm.workspace_bytes("nope", 8, 8, 8, 1)
+
+
+def _build_shim_example(tmp_path):
+    import subprocess
+    exe = str(tmp_path / "shim_example")
+    lib_dir = os.path.join(ROOT, "fast-matmul_amd", "fastmm")
+    cmd = ["g++", "-std=c++14", "-O1", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"),
+           "-I", "/opt/rocm/include", os.path.join(ROOT, "tests", "cpp", "shim_example.cpp"),
+           "-o", exe, "-L", lib_dir, "-lfastmm", "-L", "/opt/rocm/lib", "-lamdhip64",
+           "-Wl,-rpath," + lib_dir + ":/opt/rocm/lib"]
+    subprocess.check_call(cmd)
+    return exe
+
+
+def test_cpp_header_builds_against_the_library(tmp_path):
+    """include/fastmm/fast_matmul.hpp (the reference's Matrix/FastMatmul
+    spelling) compiles with a host compiler and links to libfastmm.so."""
+    exe = _build_shim_example(tmp_path)
+    assert os.path.exists(exe)

@@ -5,9 +5,11 @@ Tolerances (FP64 / FP32) are written next to each check:
   the recursion is a small dyadic rational);
 * seeded real inputs (the golden cases): the oracle first reproduces the
   reference's digest (so it IS the reference on this host), then
-  max|C_gpu - C_oracle| / (max|A| max|B| K) <= 2e-15 (FP64) / 2e-6 (FP32),
-  and the GPU's MaxRelativeDiff against classical GEMM is within a factor 8
-  of the reference's own (recorded in golden.json);
+  max|C_gpu - C_oracle| / (max|A| max|B| K) <= 2 e_ref + 1e-15 (FP64) /
+  2 e_ref + 1e-6 (FP32), e_ref being the reference's own normwise error
+  against classical GEMM on that case (golden.json, 9e-16 median, 1.2e-14 max
+  for FP64), and the GPU's MaxRelativeDiff against classical GEMM is within a
+  factor 8 of the reference's own;
 * full-size bench workload: errors against the quad oracle on sampled entries
   and a checksum identity C x = A (B x).
 """
@@ -107,11 +109,15 @@ def test_fast_matches_oracle(fm, oracle_mod, golden, case):
         assert sha(Co) == rec["sha256"], "oracle no longer reproduces the reference"
     C = _host(fm.fastmatmul(alg, _to_dev(A, fm), _to_dev(B, fm), _to_dev(C0, fm), steps, x,
                             alpha, beta))
-    tol = 2e-15 if dtype == np.float64 else 2e-6
+    # Two evaluations of the same algorithm differ by at most the sum of their
+    # errors: bound the GPU-oracle gap by twice the reference's own normwise
+    # error against classical GEMM (golden.json) plus a floor.
+    floor = 1e-15 if dtype == np.float64 else 1e-6
     scale = max(1.0, abs(alpha)) * (1 + abs(beta))
+    tol = 2 * rec["normwise"] + floor * scale
     if "approx" in alg:
         tol = 1e-6 if steps == 1 else 1e-3
-    assert _normwise(C, Co, A, B) <= tol * scale
+    assert _normwise(C, Co, A, B) <= tol
     D = oracle_mod.gemm(A.astype(np.float64), B.astype(np.float64),
                         None if C0 is None else C0.astype(np.float64), alpha, beta)
     err = oracle_mod.max_rel_diff(C.astype(np.float64), D)
@@ -191,3 +197,15 @@ def test_bench_workload_full_size(fm, oracle_mod):
     got = _host(C)[rows, cols]
     # componentwise: |C - AB| <= c * u * (|A||B|)_ij with c well below K
     assert np.all(np.abs(got - q) <= 64 * 2 ** -53 * absab)
+
+
+def test_cpp_header_program(tmp_path):
+    """The C++ drop-in header, run on the GPU: exact products through
+    strassen/grey424_26_206/smirnov333_23_139/grey423_20_144, views included."""
+    import subprocess
+    from test_capi import _build_shim_example
+    exe = _build_shim_example(tmp_path)
+    r = subprocess.run([exe], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
+                       timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout
