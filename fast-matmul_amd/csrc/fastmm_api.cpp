@@ -73,6 +73,7 @@ struct Launch {
 struct Plan {
   const FmAlgDesc* alg = nullptr;
   fm_dtype dtype = FM_F64;
+  fm_mode mode = FM_MODE_PLAIN;
   std::vector<Launch> launches;
   std::vector<char> host_buf;
   void* dbuf = nullptr;
@@ -110,8 +111,13 @@ int g_tile_override = -1;  // fm_set_gemm_tile
 // time over the configurations, where a configuration's throughput per tile
 // (eff, measured on MI355X at 8192^3) and its resident work groups per CU
 // set how many rounds of tiles the launch takes.
-int pick_tile(fm_dtype dt, int64_t M, int64_t N, int64_t count) {
+int pick_tile(fm_dtype dt, fm_mode mode, int64_t M, int64_t N, int64_t count) {
   if (g_tile_override >= 0) return g_tile_override;
+  if (mode == FM_MODE_SPLIT_BF16) {
+    // 128x128 (4 waves) unless the launch has too few tiles to fill the chip
+    const int64_t big = ((M + 127) / 128) * ((N + 127) / 128) * count;
+    return big >= 512 ? FM_TILE_128x128 : FM_TILE_64x64;
+  }
   struct C {
     int cfg;
     double eff;
@@ -235,11 +241,12 @@ void peel_descs(const Node& nd, const Dims& D, int am, int ak, int an, double be
   }
 }
 
-fm_status build_plan(Plan& P, const FmAlgDesc* alg, fm_dtype dt, int64_t M, int64_t K,
-                     int64_t N, int64_t lda, int64_t ldb, int64_t ldc, int steps, double x,
-                     double alpha, double beta) {
+fm_status build_plan(Plan& P, const FmAlgDesc* alg, fm_dtype dt, fm_mode mode, int64_t M,
+                     int64_t K, int64_t N, int64_t lda, int64_t ldb, int64_t ldc, int steps,
+                     double x, double alpha, double beta) {
   P.alg = alg;
   P.dtype = dt;
+  P.mode = mode;
   const int L = steps;
   std::vector<Dims> dims(L + 1);
   dims[0] = {M, K, N};
@@ -327,7 +334,7 @@ fm_status build_plan(Plan& P, const FmAlgDesc* alg, fm_dtype dt, int64_t M, int6
       g.beta = 0.0;
       descs.push_back(g);
     }
-    add_gemm_launch(P, descs, pick_tile(dt, d.m, d.n, (int64_t)descs.size()), true);
+    add_gemm_launch(P, descs, pick_tile(dt, P.mode, d.m, d.n, (int64_t)descs.size()), true);
   }
   for (int l = L - 1; l >= 0; --l) {
     const Dims c = dims[l + 1];
@@ -337,16 +344,17 @@ fm_status build_plan(Plan& P, const FmAlgDesc* alg, fm_dtype dt, int64_t M, int6
     add_stage_launch(P, LK_W, ops, c.m, c.n, lb[l].m_off, lb[l].m_ld, lb[l].m_slab, x, bl);
     std::vector<FmGemmDesc> peel;
     for (const Node& nd : lv[l]) peel_descs(nd, dims[l], alg->m, alg->k, alg->n, bl, peel);
-    if (!peel.empty()) add_gemm_launch(P, peel, FM_TILE_64x64, false);
+    if (!peel.empty()) add_gemm_launch(P, peel, FM_TILE_64x64, false);  // split mode: 64x64 too
   }
   P.ws_bytes = (size_t)ws * elem_size(dt);
   return FM_OK;
 }
 
-fm_status build_gemm_plan(Plan& P, fm_dtype dt, int64_t M, int64_t K, int64_t N, int64_t lda,
-                          int64_t ldb, int64_t ldc, double alpha, double beta) {
+fm_status build_gemm_plan(Plan& P, fm_dtype dt, fm_mode mode, int64_t M, int64_t K, int64_t N,
+                          int64_t lda, int64_t ldb, int64_t ldc, double alpha, double beta) {
   P.alg = nullptr;
   P.dtype = dt;
+  P.mode = mode;
   FmGemmDesc g;
   memset(&g, 0, sizeof(g));
   g.a = op_make(FM_ROLE_A, 0, lda);
@@ -358,7 +366,7 @@ fm_status build_gemm_plan(Plan& P, fm_dtype dt, int64_t M, int64_t K, int64_t N,
   g.alpha = round_scalar(alpha, dt);
   g.beta = round_scalar(beta, dt);
   std::vector<FmGemmDesc> descs(1, g);
-  add_gemm_launch(P, descs, pick_tile(dt, M, N, 1), true);
+  add_gemm_launch(P, descs, pick_tile(dt, P.mode, M, N, 1), true);
   P.ws_bytes = 0;
   return FM_OK;
 }
@@ -420,7 +428,10 @@ fm_status run_plan(const Plan& P, const void* A, const void* B, void* C, void* w
       FmGemmArgs g = L.gm;
       g.bases = bases;
       g.vec = (L.vec_ok && aligned) ? 1 : 0;
-      e = P.dtype == FM_F64 ? fm_gemm_launch_f64(L.cfg, g, s) : fm_gemm_launch_f32(L.cfg, g, s);
+      if (P.mode == FM_MODE_SPLIT_BF16)
+        e = fm_gemm_launch_bf16x3(L.cfg, g, s);
+      else
+        e = P.dtype == FM_F64 ? fm_gemm_launch_f64(L.cfg, g, s) : fm_gemm_launch_f32(L.cfg, g, s);
     } else {
       FmStageArgs a = L.st;
       a.bases = bases;
@@ -479,6 +490,14 @@ fm_status check_args(fm_dtype dt, int m, int k, int n, const void* A, int64_t ld
   return FM_OK;
 }
 
+fm_status check_mode(fm_dtype dt, fm_mode mode) {
+  if (mode == FM_MODE_PLAIN) return FM_OK;
+  if (mode == FM_MODE_SPLIT_BF16 && dt == FM_F32) return FM_OK;
+  return set_err(FM_ERR_UNSUPPORTED, mode == FM_MODE_SPLIT_BF16
+                                         ? "split_bf16 leaves need FP32 data"
+                                         : "this leaf mode is not available for this dtype");
+}
+
 // Find or build the plan; returns it through *out.
 fm_status get_plan(const char* alg_name, const FmAlgDesc* alg, fm_dtype dt, fm_mode mode, int m,
                    int k, int n, int64_t lda, int64_t ldb, int64_t ldc, int steps, double x,
@@ -494,8 +513,8 @@ fm_status get_plan(const char* alg_name, const FmAlgDesc* alg, fm_dtype dt, fm_m
     return FM_OK;
   }
   std::unique_ptr<Plan> P(new Plan());
-  fm_status st = alg ? build_plan(*P, alg, dt, m, k, n, lda, ldb, ldc, steps, x, alpha, beta)
-                     : build_gemm_plan(*P, dt, m, k, n, lda, ldb, ldc, alpha, beta);
+  fm_status st = alg ? build_plan(*P, alg, dt, mode, m, k, n, lda, ldb, ldc, steps, x, alpha, beta)
+                     : build_gemm_plan(*P, dt, mode, m, k, n, lda, ldb, ldc, alpha, beta);
   if (st != FM_OK) return st;
   st = upload_plan(*P);
   if (st != FM_OK) return st;
@@ -570,8 +589,8 @@ fm_status fm_fastmatmul(const char* alg_name, fm_dtype dtype, fm_mode mode, int 
   fm_status st = check_args(dtype, m, k, n, A, lda, B, ldb, C, ldc);
   if (st != FM_OK) return st;
   if (num_steps < 0) return set_err(FM_ERR_INVALID_ARG, "negative num_steps");
-  if (mode != FM_MODE_PLAIN)
-    return set_err(FM_ERR_UNSUPPORTED, "this leaf mode is not available for this dtype");
+  st = check_mode(dtype, mode);
+  if (st != FM_OK) return st;
   if (m == 0 || n == 0) return FM_OK;
   if (k == 0) return set_err(FM_ERR_SHAPE, "k == 0");
   std::lock_guard<std::mutex> lock(g_mu);
@@ -600,26 +619,28 @@ fm_status fm_workspace_query(const char* alg_name, fm_dtype dtype, fm_mode mode,
   if (!alg) return set_err(FM_ERR_UNKNOWN_ALG, "unknown algorithm");
   if (m <= 0 || k <= 0 || n <= 0 || num_steps < 0)
     return set_err(FM_ERR_INVALID_ARG, "bad shape");
-  if (mode != FM_MODE_PLAIN) return set_err(FM_ERR_UNSUPPORTED, "mode not available");
+  fm_status st = check_mode(dtype, mode);
+  if (st != FM_OK) return st;
   Plan P;
-  fm_status st = build_plan(P, alg, dtype, m, k, n, m, k, m, num_steps, 1e-8, 1.0, 0.0);
+  st = build_plan(P, alg, dtype, mode, m, k, n, m, k, m, num_steps, 1e-8, 1.0, 0.0);
   if (st != FM_OK) return st;
   *bytes = P.ws_bytes;
   return FM_OK;
 }
 
-fm_status fm_gemm(fm_dtype dtype, int m, int k, int n, const void* A, int64_t lda,
+fm_status fm_gemm(fm_dtype dtype, fm_mode mode, int m, int k, int n, const void* A, int64_t lda,
                   const void* B, int64_t ldb, void* C, int64_t ldc, double alpha, double beta,
                   void* stream) {
   g_err.clear();
   fm_status st = check_args(dtype, m, k, n, A, lda, B, ldb, C, ldc);
   if (st != FM_OK) return st;
+  st = check_mode(dtype, mode);
+  if (st != FM_OK) return st;
   if (m == 0 || n == 0) return FM_OK;
   if (k == 0) return set_err(FM_ERR_SHAPE, "k == 0");
   std::lock_guard<std::mutex> lock(g_mu);
   Plan* P = nullptr;
-  st = get_plan(nullptr, nullptr, dtype, FM_MODE_PLAIN, m, k, n, lda, ldb, ldc, 0, 0.0, alpha,
-                beta, &P);
+  st = get_plan(nullptr, nullptr, dtype, mode, m, k, n, lda, ldb, ldc, 0, 0.0, alpha, beta, &P);
   if (st != FM_OK) return st;
   return run_plan(*P, A, B, C, nullptr, (hipStream_t)stream);
 }

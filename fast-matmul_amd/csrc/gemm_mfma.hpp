@@ -58,3 +58,5 @@ inline void fm_tile_dims(int cfg, int* bm, int* bn) {
 
 hipError_t fm_gemm_launch_f64(int cfg, const FmGemmArgs& a, hipStream_t s);
 hipError_t fm_gemm_launch_f32(int cfg, const FmGemmArgs& a, hipStream_t s);
+// FP32 data, split-bf16 slice products (csrc/gemm_split.hip)
+hipError_t fm_gemm_launch_bf16x3(int cfg, const FmGemmArgs& a, hipStream_t s);

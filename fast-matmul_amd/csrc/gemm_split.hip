@@ -1,0 +1,231 @@
+// gemm_split.hip -- FP32-accurate leaf products from split bfloat16 slices.
+//
+// Each FP32 operand value x is split on its way into LDS into three bf16
+// slices x = x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1);
+// every difference is exact in FP32, and the three 8-bit significands carry
+// the 24 bits of x).  A B is then the sum of the slice products A_i B_j with
+// i + j <= 2 (six v_mfma_f32_16x16x32_bf16 per output fragment and K step,
+// smallest terms first), accumulated in FP32: the dropped terms are below
+// 2^-24 relative to |A||B|, the accumulation error is that of an FP32 GEMM.
+// The leaf contract is the reference's (linalg/linalg.hpp:261-269):
+// C = alpha * A B + beta * C, column-major, grouped over many products.
+//
+// LDS holds each slice plane with K contiguous per row of A and per column
+// of B (rows padded to 40 bf16 = 80 B so eight lanes' 16-byte reads hit
+// distinct banks).  One LDS buffer, register prefetch of the next K slice,
+// two barriers per slice.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gemm_mfma.hpp"
+
+typedef __bf16 fm_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float fm_f32x4 __attribute__((ext_vector_type(4)));
+
+// round-to-nearest-even float -> bf16 (inputs are finite)
+__device__ __forceinline__ uint16_t fm_bf16_bits(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float fm_bf16_to_f32(uint16_t b) {
+  return __uint_as_float((uint32_t)b << 16);
+}
+
+// split 8 values into three slice planes (8 bf16 each, packed in 4 uints)
+__device__ __forceinline__ void fm_split3(const float (&x)[8], uint4 (&out)[3]) {
+  uint32_t h[8], m[8], l[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const uint16_t b0 = fm_bf16_bits(x[e]);
+    const float r1 = __fsub_rn(x[e], fm_bf16_to_f32(b0));
+    const uint16_t b1 = fm_bf16_bits(r1);
+    const float r2 = __fsub_rn(r1, fm_bf16_to_f32(b1));
+    const uint16_t b2 = fm_bf16_bits(r2);
+    h[e] = b0;
+    m[e] = b1;
+    l[e] = b2;
+  }
+  out[0] = make_uint4(h[0] | h[1] << 16, h[2] | h[3] << 16, h[4] | h[5] << 16, h[6] | h[7] << 16);
+  out[1] = make_uint4(m[0] | m[1] << 16, m[2] | m[3] << 16, m[4] | m[5] << 16, m[6] | m[7] << 16);
+  out[2] = make_uint4(l[0] | l[1] << 16, l[2] | l[3] << 16, l[4] | l[5] << 16, l[6] | l[7] << 16);
+}
+
+__device__ __forceinline__ int fm_xcd_remap2(int wg, int nwg) {
+  const int full = nwg & ~7;
+  if (wg < full) return (wg & 7) * (full >> 3) + (wg >> 3);
+  return wg;
+}
+
+template <int BM, int BN, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN) fm_gemm_bf16x3_kernel(FmGemmArgs args) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int WTM = BM / WM, WTN = BN / WN;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  constexpr int BK = 32, KP = BK + 8;            // bf16 per padded LDS row
+  constexpr int A_OCT = BM * BK / 8 / NT;         // 8-element octets per thread
+  constexpr int B_OCT = BN * BK / 8 / NT;
+  static_assert(A_OCT * NT * 8 == BM * BK && B_OCT * NT * 8 == BN * BK, "tile/thread mismatch");
+  static_assert(TM >= 1 && TN >= 1, "bad tile");
+
+  __shared__ __align__(16) uint16_t As[3][BM][KP];
+  __shared__ __align__(16) uint16_t Bs[3][BN][KP];
+
+  const int wg = fm_xcd_remap2(blockIdx.x, gridDim.x);
+  int g;
+  if (args.uniform) {
+    g = wg / (args.descs[0].tiles_m * args.descs[0].tiles_n);
+  } else {
+    int lo = 0, hi = args.ndesc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (args.descs[mid].tile_begin <= wg)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    g = lo;
+  }
+  const FmGemmDesc d = args.descs[g];
+  const int local = wg - d.tile_begin;
+  const int m0 = (local % d.tiles_m) * BM, n0 = (local / d.tiles_m) * BN;
+  const int M = d.M, N = d.N, K = d.K;
+  const float* __restrict__ A = reinterpret_cast<const float*>(args.bases.p[d.a.role]) + d.a.off;
+  const float* __restrict__ B = reinterpret_cast<const float*>(args.bases.p[d.b.role]) + d.b.off;
+  float* __restrict__ C = reinterpret_cast<float*>(args.bases.p[d.c.role]) + d.c.off;
+  const int64_t lda = d.a.ld, ldb = d.b.ld, ldc = d.c.ld;
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+
+  float ra[A_OCT][8], rb[B_OCT][8];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < A_OCT; ++u) {
+      const int o = tid + u * NT;
+      const int row = o % BM, kq = o / BM;
+      const bool rok = m0 + row < M;
+      const float* p = A + (m0 + row) + (int64_t)(k0 + 8 * kq) * lda;
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        ra[u][e] = (rok && k0 + 8 * kq + e < K) ? p[(int64_t)e * lda] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < B_OCT; ++u) {
+      const int o = tid + u * NT;
+      const int kq = o % (BK / 8), col = o / (BK / 8);
+      const bool cok = n0 + col < N;
+      const float* p = B + (k0 + 8 * kq) + (int64_t)(n0 + col) * ldb;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) rb[u][e] = (cok && k0 + 8 * kq + e < K) ? p[e] : 0.f;
+    }
+  };
+  auto sstore = [&]() {
+#pragma unroll
+    for (int u = 0; u < A_OCT; ++u) {
+      const int o = tid + u * NT;
+      const int row = o % BM, kq = o / BM;
+      uint4 sl[3];
+      fm_split3(ra[u], sl);
+#pragma unroll
+      for (int p = 0; p < 3; ++p) *reinterpret_cast<uint4*>(&As[p][row][8 * kq]) = sl[p];
+    }
+#pragma unroll
+    for (int u = 0; u < B_OCT; ++u) {
+      const int o = tid + u * NT;
+      const int kq = o % (BK / 8), col = o / (BK / 8);
+      uint4 sl[3];
+      fm_split3(rb[u], sl);
+#pragma unroll
+      for (int p = 0; p < 3; ++p) *reinterpret_cast<uint4*>(&Bs[p][col][8 * kq]) = sl[p];
+    }
+  };
+
+  fm_f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = fm_f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int nk = (K + BK - 1) / BK;
+  const int kofs = 8 * (lane >> 4);
+  gload(0);
+  for (int kt = 0; kt < nk; ++kt) {
+    sstore();
+    __syncthreads();
+    if (kt + 1 < nk) gload((kt + 1) * BK);
+    fm_bf16x8 af[3][TM], bf[3][TN];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+        af[p][i] = *reinterpret_cast<const fm_bf16x8*>(&As[p][wm * WTM + i * 16 + (lane & 15)][kofs]);
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        bf[p][j] = *reinterpret_cast<const fm_bf16x8*>(&Bs[p][wn * WTN + j * 16 + (lane & 15)][kofs]);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        // slice products A_a B_b with a + b <= 2, smallest first; the MFMA's
+        // first operand is the B slice (rows = C columns), its second the A slice
+        fm_f32x4 c = acc[i][j];
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[2][j], af[0][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[1][j], af[1][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[2][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[1][j], af[0][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[1][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[0][i], c, 0, 0, 0);
+        acc[i][j] = c;
+      }
+    __syncthreads();
+  }
+
+  const float alpha = (float)d.alpha, beta = (float)d.beta;
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int row = m0 + wm * WTM + i * 16 + (lane & 15);
+    if (row >= M) continue;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = n0 + wn * WTN + j * 16 + 4 * (lane >> 4) + v;
+        if (col < N) {
+          float* p = C + row + (int64_t)col * ldc;
+          float r = alpha * acc[i][j][v];
+          if (beta != 0.f) r += beta * *p;
+          *p = r;
+        }
+      }
+    }
+  }
+}
+
+hipError_t fm_gemm_launch_bf16x3(int cfg, const FmGemmArgs& a, hipStream_t s) {
+  if (a.total_tiles <= 0) return hipSuccess;
+  switch (cfg) {
+    case FM_TILE_64x64:
+      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<64, 64, 2, 2>), dim3(a.total_tiles), dim3(256), 0,
+                         s, a);
+      break;
+    case FM_TILE_128x128_W8:
+      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<128, 128, 2, 4>), dim3(a.total_tiles),
+                         dim3(512), 0, s, a);
+      break;
+    case FM_TILE_128x64:
+      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<128, 64, 2, 2>), dim3(a.total_tiles), dim3(256),
+                         0, s, a);
+      break;
+    case FM_TILE_64x128:
+      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<64, 128, 2, 2>), dim3(a.total_tiles), dim3(256),
+                         0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<128, 128, 2, 2>), dim3(a.total_tiles),
+                         dim3(256), 0, s, a);
+      break;
+  }
+  return hipGetLastError();
+}

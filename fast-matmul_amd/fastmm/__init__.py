@@ -105,8 +105,8 @@ def fastmatmul(alg_name: str, A: torch.Tensor, B: torch.Tensor, C: Optional[torc
 
 
 def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alpha: float = 1.0,
-         beta: float = 0.0) -> torch.Tensor:
-    """Classical C := alpha * A @ B + beta * C on the library's MFMA kernel."""
+         beta: float = 0.0, mode: str = "plain") -> torch.Tensor:
+    """Classical C := alpha * A @ B + beta * C on the library's leaf kernel."""
     A = _col_major(A, "A")
     B = _col_major(B, "B")
     m, k = A.shape
@@ -114,7 +114,7 @@ def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alp
     if C is None:
         C = empty(m, n, A.dtype, A.device)
     C = _col_major(C, "C")
-    st = LIB.fm_gemm(_DT[A.dtype], m, k, n, ctypes.c_void_p(A.data_ptr()), _ld(A),
+    st = LIB.fm_gemm(_DT[A.dtype], _lib.MODES[mode], m, k, n, ctypes.c_void_p(A.data_ptr()), _ld(A),
                      ctypes.c_void_p(B.data_ptr()), _ld(B), ctypes.c_void_p(C.data_ptr()), _ld(C),
                      float(alpha), float(beta), _stream_handle(A))
     check(st)

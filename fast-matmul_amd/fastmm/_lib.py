@@ -60,8 +60,8 @@ def _load():
     lib.fm_workspace_query.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int,
                                        c_int, ctypes.POINTER(ctypes.c_size_t)]
     lib.fm_gemm.restype = c_int
-    lib.fm_gemm.argtypes = [c_int, c_int, c_int, c_int, vp, i64, vp, i64, vp, i64, c_double,
-                            c_double, vp]
+    lib.fm_gemm.argtypes = [c_int, c_int, c_int, c_int, c_int, vp, i64, vp, i64, vp, i64,
+                            c_double, c_double, vp]
     lib.fm_num_algorithms.restype = c_int
     lib.fm_algorithm_name.restype = ctypes.c_char_p
     lib.fm_algorithm_name.argtypes = [c_int]

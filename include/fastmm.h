@@ -77,10 +77,10 @@ fm_status fm_fastmatmul(const char* alg, fm_dtype dtype, fm_mode mode, int m, in
 fm_status fm_workspace_query(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k,
                              int n, int num_steps, size_t* bytes);
 
-/* Classical C := alpha * A * B + beta * C on the same MFMA kernels
- * (MatMul, linalg/linalg.hpp:261-269). */
-fm_status fm_gemm(fm_dtype dtype, int m, int k, int n, const void* A, int64_t lda,
-                  const void* B, int64_t ldb, void* C, int64_t ldc, double alpha,
+/* Classical C := alpha * A * B + beta * C on the same leaf kernels
+ * (MatMul, linalg/linalg.hpp:261-269), with the same leaf modes. */
+fm_status fm_gemm(fm_dtype dtype, fm_mode mode, int m, int k, int n, const void* A,
+                  int64_t lda, const void* B, int64_t ldb, void* C, int64_t ldc, double alpha,
                   double beta, void* stream);
 
 /* Registry (algorithms/all_algorithms.hpp:52-90, 247-326). */

@@ -171,7 +171,7 @@ double run(const char* alg, Matrix<Scalar>& A, Matrix<Scalar>& B, Matrix<Scalar>
 template <typename Scalar>
 void MatMul(Matrix<Scalar>& A, Matrix<Scalar>& B, Matrix<Scalar>& C, Scalar beta = Scalar(0)) {
   detail::DevMat<Scalar> dA(A, true), dB(B, true), dC(C, beta != Scalar(0));
-  detail::check(fm_gemm(detail::dtype_of<Scalar>(), A.m(), A.n(), B.n(), dA.p,
+  detail::check(fm_gemm(detail::dtype_of<Scalar>(), FM_MODE_PLAIN, A.m(), A.n(), B.n(), dA.p,
                         dA.m > 0 ? dA.m : 1, dB.p, dB.m > 0 ? dB.m : 1, dC.p,
                         dC.m > 0 ? dC.m : 1, (double)C.multiplier(), (double)beta, nullptr));
   detail::hcheck(hipDeviceSynchronize());

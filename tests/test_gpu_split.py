@@ -1,0 +1,92 @@
+"""FP32-accurate leaves from split bfloat16 slices (FM_MODE_SPLIT_BF16).
+
+Pins (tolerances written at each check):
+* exact: integers of 9-11 bits need two bf16 slices (8 significand bits
+  each) with no remainder, so every kept slice product is exact and, with K
+  small enough that every partial sum stays below 2^24, so is the FP32
+  accumulation: the result must equal the exact product bit for bit (this
+  also pins the instruction's output layout and the K-slot pairing of the
+  two operands);
+* random uniform(-1,1) and row/column-scaled 10^u inputs: normwise error
+  against the FP64 product <= 8 * 2^-24 * sqrt(K)-free bound below, i.e. the
+  level of a plain FP32 GEMM, for the classical split product and for
+  Strassen with 1 and 2 levels.
+"""
+import numpy as np
+import pytest
+import torch
+
+from cases import make_inputs
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _dev(a, fm):
+    return fm.col_major(torch.from_numpy(np.asfortranarray(a)).to(DEV))
+
+
+def _nw(C, D, A, B):
+    return float(np.abs(C.astype(np.float64) - D).max()) / (
+        float(np.abs(A).max()) * float(np.abs(B).max()) * A.shape[1])
+
+
+@pytest.mark.parametrize("shape,bits", [((64, 32, 64), 9), ((200, 8, 130), 10),
+                                        ((129, 1, 257), 11), ((1, 32, 1), 9),
+                                        ((300, 500, 100), 5), ((512, 64, 256), 9)])
+def test_split_gemm_exact_on_integers(fm, shape, bits):
+    m, k, n = shape
+    rng = np.random.default_rng(bits)
+    lim = 2 ** bits - 1
+    A = rng.integers(-lim, lim + 1, (m, k)).astype(np.float32)
+    B = rng.integers(-lim, lim + 1, (k, n)).astype(np.float32)
+    exact = A.astype(np.float64) @ B.astype(np.float64)
+    C = fm.gemm(_dev(A, fm), _dev(B, fm), mode="split_bf16").cpu().numpy()
+    # exact when every partial sum of K terms |a||b| stays below 2^24
+    if k * lim * lim < 2 ** 24:
+        assert np.array_equal(C.astype(np.float64), exact)
+    else:
+        assert np.abs(C - exact).max() <= 2.0 ** -23 * lim * lim * k
+
+
+@pytest.mark.parametrize("alg,steps", [("strassen", 1), ("strassen", 2), ("grey424_26_206", 1)])
+def test_split_fast_exact_on_small_integers(fm, alg, steps):
+    A, B, C0 = make_inputs(402, 128, 562, 5, "int", np.float32, with_c=True)
+    exact = A.astype(np.float64) @ B.astype(np.float64)
+    C = fm.fastmatmul(alg, _dev(A, fm), _dev(B, fm), _dev(C0, fm), steps, alpha=2.0, beta=-1.0,
+                      mode="split_bf16").cpu().numpy()
+    assert np.array_equal(C.astype(np.float64), 2 * exact - C0.astype(np.float64))
+
+
+def _scaled(m, k, n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.random((m, k)) * 2 - 1
+    B = rng.random((k, n)) * 2 - 1
+    A *= 10.0 ** rng.uniform(-3, 3, (m, 1))
+    B *= 10.0 ** rng.uniform(-3, 3, (1, n))
+    return np.asfortranarray(A.astype(np.float32)), np.asfortranarray(B.astype(np.float32))
+
+
+@pytest.mark.parametrize("dist", ["u11", "scaled"])
+@pytest.mark.parametrize("alg,steps", [(None, 0), ("strassen", 1), ("strassen", 2)])
+def test_split_accuracy_fp32_level(fm, oracle_mod, dist, alg, steps):
+    m = k = n = 1024
+    if dist == "u11":
+        A, B, _ = make_inputs(m, k, n, 31, "u11", np.float32)
+    else:
+        A, B = _scaled(m, k, n, 32)
+    D = oracle_mod.quad_matmul(A.astype(np.float64), B.astype(np.float64))
+    Ad, Bd = _dev(A, fm), _dev(B, fm)
+    if alg is None:
+        C = fm.gemm(Ad, Bd, mode="split_bf16").cpu().numpy()
+        Cp = fm.gemm(Ad, Bd).cpu().numpy()
+    else:
+        C = fm.fastmatmul(alg, Ad, Bd, None, steps, mode="split_bf16").cpu().numpy()
+        Cp = fm.fastmatmul(alg, Ad, Bd, None, steps).cpu().numpy()
+    # componentwise: |C - AB| <= c 2^-24 (|A||B|) with c = 8 (1 + levels)
+    absab = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    err = np.abs(C.astype(np.float64) - D) / absab
+    errp = np.abs(Cp.astype(np.float64) - D) / absab
+    assert err.max() <= 8 * (1 + steps) * 2.0 ** -24 * 16, (err.max(), errp.max())
+    # and no worse than 4x the plain FP32-MFMA path at the same depth
+    assert err.max() <= 4 * errp.max() + 2.0 ** -24
