@@ -69,6 +69,138 @@ def timed(fn, steps, sync):
     return (time.perf_counter() - t0) / steps
 
 
+def _rate(fn, flops, sync, steps=3):
+    fn()
+    sync()
+    t = timed(fn, steps, sync)
+    return flops / t / 1e12, t * 1e3
+
+
+def _err(C, D, A, B):
+    """(normwise, elementwise) error of C against the reference product D."""
+    d = (C.double() - D).abs()
+    nw = float(d.max()) / (float(A.abs().max()) * float(B.abs().max()) * A.shape[1])
+    return nw, float((d / D.abs()).max())
+
+
+def other_configs(fastmm, torch, dev):
+    """Secondary measurements of the other configurations, each guarded."""
+    sync = torch.cuda.synchronize
+    out = {}
+    g = torch.Generator(device=dev)
+    g.manual_seed(0xC0FFEE)
+
+    def rnd(m, n, dt):
+        return fastmm.col_major(torch.rand((m, n), generator=g, device=dev, dtype=dt) * 2 - 1)
+
+    def guarded(name, fn):
+        try:
+            out[name] = fn()
+        except Exception as e:  # report, never fail the headline number
+            out[name] = {"error": "%s: %s" % (type(e).__name__, e)}
+        torch.cuda.empty_cache()
+
+    def c1():
+        n = 2048
+        A, B = rnd(n, n, torch.float64), rnd(n, n, torch.float64)
+        C = fastmm.empty(n, n, torch.float64, dev)
+        fl = 2.0 * n ** 3
+        v, ms = _rate(lambda: fastmm.fastmatmul("strassen", A, B, C, 1), fl, sync, 10)
+        D = torch.matmul(B.t(), A.t()).t()
+        vb, _ = _rate(lambda: torch.matmul(B.t(), A.t()), fl, sync, 10)
+        nw, ew = _err(C, D, A, B)
+        return {"workload": "strassen numsteps=1 FP64 2048^3", "tflops": v, "ms": ms,
+                "torch_matmul_tflops": vb, "normwise_err": nw, "max_rel_diff": ew}
+
+    def c2ts():
+        M, K, N = 131072, 2400, 2400
+        A, B = rnd(M, K, torch.float64), rnd(K, N, torch.float64)
+        C = fastmm.empty(M, N, torch.float64, dev)
+        fl = 2.0 * M * K * N
+        r = {"workload": "grey424_26_206 FP64 131072x2400x2400"}
+        for st in (1, 2):
+            r["numsteps%d_tflops" % st], _ = _rate(
+                lambda: fastmm.fastmatmul("grey424_26_206", A, B, C, st), fl, sync)
+        r["fastmm_gemm_tflops"], _ = _rate(lambda: fastmm.gemm(A, B, C), fl, sync)
+        r["torch_matmul_tflops"], _ = _rate(lambda: torch.matmul(B.t(), A.t()), fl, sync)
+        D = torch.matmul(B.t(), A.t()).t()
+        fastmm.fastmatmul("grey424_26_206", A, B, C, 2)
+        r["normwise_err_numsteps2"], r["max_rel_diff_numsteps2"] = _err(C, D, A, B)
+        return r
+
+    def c4():
+        n = 16384
+        r = {"workload": "strassen numsteps=2 FP32 data 16384^3, split-bf16 leaves (6 products)"}
+        A, B = rnd(n, n, torch.float32), rnd(n, n, torch.float32)
+        for dist in ("uniform", "scaled"):
+            if dist == "scaled":
+                rs = 10.0 ** (torch.rand((n, 1), generator=g, device=dev, dtype=torch.float32) * 6 - 3)
+                cs = 10.0 ** (torch.rand((1, n), generator=g, device=dev, dtype=torch.float32) * 6 - 3)
+                A = fastmm.col_major((A * rs).contiguous())
+                B = fastmm.col_major((B * cs).contiguous())
+            C = fastmm.empty(n, n, torch.float32, dev)
+            fl = 2.0 * n ** 3
+            v, ms = _rate(lambda: fastmm.fastmatmul("strassen", A, B, C, 2, mode="split_bf16"), fl, sync)
+            # FP64 product as the oracle, on a 2048-row slice
+            rows = slice(0, 2048)
+            D = torch.matmul(B.double().t(), A[rows].double().t()).t()
+            nw, ew = _err(C[rows], D, A[rows].double(), B.double())
+            r[dist] = {"tflops": v, "ms": ms, "normwise_err_vs_fp64": nw, "max_rel_diff_vs_fp64": ew}
+            fastmm.fastmatmul("strassen", A, B, C, 2)
+            nwp, _ = _err(C[rows], D, A[rows].double(), B.double())
+            r[dist]["plain_fp32_leaves_normwise_err"] = nwp
+            del D
+        r["plain_fp32_strassen2_tflops"], _ = _rate(lambda: fastmm.fastmatmul("strassen", A, B, C, 2), 2.0 * n ** 3, sync)
+        r["torch_fp32_matmul_tflops"], _ = _rate(lambda: torch.matmul(B.t(), A.t()), 2.0 * n ** 3, sync)
+        return r
+
+    def c5():
+        n = 13824
+        A, B = rnd(n, n, torch.float64), rnd(n, n, torch.float64)
+        C = fastmm.empty(n, n, torch.float64, dev)
+        fl = 2.0 * n ** 3
+        r = {"workload": "smirnov333_23_139 numsteps=2 FP64 13824^3, int8-digit leaves (8 digits)"}
+        r["ozaki_tflops"], _ = _rate(
+            lambda: fastmm.fastmatmul("smirnov333_23_139", A, B, C, 2, mode="ozaki_int8"), fl, sync)
+        D = torch.matmul(B.t(), A.t()).t()
+        r["ozaki_normwise_err"], r["ozaki_max_rel_diff"] = _err(C, D, A, B)
+        r["plain_fp64_tflops"], _ = _rate(lambda: fastmm.fastmatmul("smirnov333_23_139", A, B, C, 2), fl, sync)
+        r["plain_normwise_err"], _ = _err(C, D, A, B)
+        r["torch_matmul_tflops"], _ = _rate(lambda: torch.matmul(B.t(), A.t()), fl, sync)
+        return r
+
+    def c3():
+        import numpy as np
+        rng = np.random.default_rng(0xC3)
+        vals = np.arange(512, 4097, 64)
+        shapes = [tuple(int(v) for v in rng.choice(vals, 3)) for _ in range(256)]
+        probs = []
+        fl = 0.0
+        for (m, k, n) in shapes:
+            probs.append({"A": rnd(m, k, torch.float32), "B": rnd(k, n, torch.float32),
+                          "C": fastmm.empty(m, n, torch.float32, dev)})
+            fl += 2.0 * m * k * n
+        b = fastmm.Batch(probs, graph=True)
+        v, ms = _rate(b.run, fl, sync)
+        chosen = {}
+        for i in range(len(probs)):
+            a, st = b.choice(i)
+            chosen["%s/%d" % (a, st)] = chosen.get("%s/%d" % (a, st), 0) + 1
+        # classical reference: one torch matmul per problem
+        vt, _ = _rate(lambda: [torch.matmul(p["B"].t(), p["A"].t()) for p in probs], fl, sync)
+        errs = []
+        for p in probs[:8]:
+            D = torch.matmul(p["B"].double().t(), p["A"].double().t()).t()
+            errs.append(_err(p["C"], D, p["A"].double(), p["B"].double())[0])
+        return {"workload": "256 FP32 problems, M,K,N uniform on {512,...,4096}/64, autotuned, hipGraph",
+                "total_gflop": fl / 1e9, "tflops": v, "ms": ms, "choices": chosen,
+                "torch_per_problem_matmul_tflops": vt, "normwise_err_max_first8": max(errs)}
+
+    for name, fn in (("c1", c1), ("c2ts", c2ts), ("c3", c3), ("c4", c4), ("c5", c5)):
+        guarded(name, fn)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +209,8 @@ def main():
     ap.add_argument("--workload", default="c2sq", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the host CPU baseline")
     ap.add_argument("--cpu-n", type=int, default=4096, help="square size of the CPU sample")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the secondary measurements of the other workloads")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB read + 1 GiB write elementwise kernel, the "
                          "known-size reference for HBM byte counters")
@@ -143,6 +277,12 @@ def main():
         sync()
         del x, y
 
+    configs = None
+    if not args.no_configs and args.workload == "c2sq":
+        del Cg, D, At, Bt
+        torch.cuda.empty_cache()
+        configs = other_configs(fastmm, torch, dev)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -189,6 +329,7 @@ def main():
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)
                      if args.workload == "c2sq" else None},
         "cpu_baseline": cpu,
+        "configs": configs,
     }
     print(json.dumps(out))
 

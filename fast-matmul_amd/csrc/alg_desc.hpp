@@ -13,9 +13,16 @@
 
 enum FmRole : int32_t { FM_ROLE_A = 0, FM_ROLE_B = 1, FM_ROLE_C = 2, FM_ROLE_WS = 3 };
 
+// Roles 0-3 come from p[]; a batched plan numbers its problems' operands
+// from 4 on and resolves them through a device table (ext[role - 4]).
 struct FmBases {
   void* p[4];
+  void* const* ext;
 };
+
+__host__ __device__ __forceinline__ void* fm_base(const FmBases& b, int32_t role) {
+  return role < 4 ? b.p[role] : b.ext[role - 4];
+}
 
 // A column-major operand: element (i, j) at base[role] + off + i + j * ld.
 struct FmOp {

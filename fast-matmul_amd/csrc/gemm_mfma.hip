@@ -79,7 +79,10 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
   __shared__ __align__(16) T As[2][BK][BM];
   __shared__ __align__(16) T Bs[2][BN][BKP];
 
-  const int wg = fm_xcd_remap(blockIdx.x, gridDim.x);
+  // Persistent when the grid is smaller than the tile count: work group b
+  // takes tiles b, b + grid, b + 2 grid, ...
+  for (int tile = blockIdx.x; tile < args.total_tiles; tile += gridDim.x) {
+  const int wg = fm_xcd_remap(tile, args.total_tiles);
   int g;
   if (args.uniform) {
     const int per = args.descs[0].tiles_m * args.descs[0].tiles_n;
@@ -100,9 +103,9 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
   const int tmi = local % d.tiles_m, tni = local / d.tiles_m;
   const int m0 = tmi * BM, n0 = tni * BN;
   const int M = d.M, N = d.N, K = d.K;
-  const T* __restrict__ A = reinterpret_cast<const T*>(args.bases.p[d.a.role]) + d.a.off;
-  const T* __restrict__ B = reinterpret_cast<const T*>(args.bases.p[d.b.role]) + d.b.off;
-  T* __restrict__ C = reinterpret_cast<T*>(args.bases.p[d.c.role]) + d.c.off;
+  const T* __restrict__ A = reinterpret_cast<const T*>(fm_base(args.bases, d.a.role)) + d.a.off;
+  const T* __restrict__ B = reinterpret_cast<const T*>(fm_base(args.bases, d.b.role)) + d.b.off;
+  T* __restrict__ C = reinterpret_cast<T*>(fm_base(args.bases, d.c.role)) + d.c.off;
   const int64_t lda = d.a.ld, ldb = d.b.ld, ldc = d.c.ld;
 
   const int tid = threadIdx.x;
@@ -244,12 +247,14 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
       }
     }
   }
+  }  // tile loop
 }
 
 template <typename T, int BM, int BN, int BK, int WM, int WN>
 static void fm_gemm_go(const FmGemmArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL((fm_gemm_kernel<T, BM, BN, BK, WM, WN>), dim3(a.total_tiles),
-                     dim3(64 * WM * WN), 0, s, a);
+  const int grid = a.grid > 0 && a.grid < a.total_tiles ? a.grid : a.total_tiles;
+  hipLaunchKernelGGL((fm_gemm_kernel<T, BM, BN, BK, WM, WN>), dim3(grid), dim3(64 * WM * WN), 0, s,
+                     a);
 }
 
 template <typename T>

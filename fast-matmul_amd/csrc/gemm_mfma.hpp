@@ -24,6 +24,8 @@ struct FmGemmArgs {
   int32_t total_tiles;
   int32_t uniform;  // all products have one shape: tile -> product by division
   int32_t vec;      // every A/B column start is 16-byte aligned: vector loads allowed
+  int32_t grid;     // > 0: persistent launch with this many work groups
+  int32_t pad;
 };
 
 // Tile configurations of the leaf kernel: BM x BN output tile, BK-deep K

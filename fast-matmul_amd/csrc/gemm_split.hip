@@ -71,7 +71,9 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_bf16x3_kernel(FmGemmArgs
   __shared__ __align__(16) uint16_t As[3][BM][KP];
   __shared__ __align__(16) uint16_t Bs[3][BN][KP];
 
-  const int wg = fm_xcd_remap2(blockIdx.x, gridDim.x);
+  // persistent when the grid is smaller than the tile count
+  for (int tile = blockIdx.x; tile < args.total_tiles; tile += gridDim.x) {
+  const int wg = fm_xcd_remap2(tile, args.total_tiles);
   int g;
   if (args.uniform) {
     g = wg / (args.descs[0].tiles_m * args.descs[0].tiles_n);
@@ -90,9 +92,9 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_bf16x3_kernel(FmGemmArgs
   const int local = wg - d.tile_begin;
   const int m0 = (local % d.tiles_m) * BM, n0 = (local / d.tiles_m) * BN;
   const int M = d.M, N = d.N, K = d.K;
-  const float* __restrict__ A = reinterpret_cast<const float*>(args.bases.p[d.a.role]) + d.a.off;
-  const float* __restrict__ B = reinterpret_cast<const float*>(args.bases.p[d.b.role]) + d.b.off;
-  float* __restrict__ C = reinterpret_cast<float*>(args.bases.p[d.c.role]) + d.c.off;
+  const float* __restrict__ A = reinterpret_cast<const float*>(fm_base(args.bases, d.a.role)) + d.a.off;
+  const float* __restrict__ B = reinterpret_cast<const float*>(fm_base(args.bases, d.b.role)) + d.b.off;
+  float* __restrict__ C = reinterpret_cast<float*>(fm_base(args.bases, d.c.role)) + d.c.off;
   const int64_t lda = d.a.ld, ldb = d.b.ld, ldc = d.c.ld;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -201,29 +203,31 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_bf16x3_kernel(FmGemmArgs
       }
     }
   }
+  }  // tile loop
 }
 
 hipError_t fm_gemm_launch_bf16x3(int cfg, const FmGemmArgs& a, hipStream_t s) {
   if (a.total_tiles <= 0) return hipSuccess;
+  const int grid = a.grid > 0 && a.grid < a.total_tiles ? a.grid : a.total_tiles;
   switch (cfg) {
     case FM_TILE_64x64:
-      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<64, 64, 2, 2>), dim3(a.total_tiles), dim3(256), 0,
+      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<64, 64, 2, 2>), dim3(grid), dim3(256), 0,
                          s, a);
       break;
     case FM_TILE_128x128_W8:
-      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<128, 128, 2, 4>), dim3(a.total_tiles),
+      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<128, 128, 2, 4>), dim3(grid),
                          dim3(512), 0, s, a);
       break;
     case FM_TILE_128x64:
-      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<128, 64, 2, 2>), dim3(a.total_tiles), dim3(256),
+      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<128, 64, 2, 2>), dim3(grid), dim3(256),
                          0, s, a);
       break;
     case FM_TILE_64x128:
-      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<64, 128, 2, 2>), dim3(a.total_tiles), dim3(256),
+      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<64, 128, 2, 2>), dim3(grid), dim3(256),
                          0, s, a);
       break;
     default:
-      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<128, 128, 2, 2>), dim3(a.total_tiles),
+      hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<128, 128, 2, 2>), dim3(grid),
                          dim3(256), 0, s, a);
       break;
   }

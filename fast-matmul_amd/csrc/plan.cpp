@@ -1,0 +1,506 @@
+// plan.cpp -- plan builder and executor.
+//
+// A sub-plan is the reference's recursion (algorithms/strassen.hpp:378-659)
+// unrolled level by level.  Level l holds R^l nodes, each an (A, B, C)
+// operand triple plus the node's multiplier.  Every node of a level has the
+// same block shape (the floor split of linalg/linalg.hpp:179-184), so each
+// level is one S and one T launch over all of its nodes; the base-case
+// products of every node become one grouped GEMM launch; then, deepest level
+// first, one W launch and one grouped launch of the peeling products.  This
+// is the reference's BFS schedule (util/par_util.hpp:17-95) with every task
+// of a level in one launch.
+//
+// Multipliers follow the reference exactly: the root carries alpha, a
+// product whose S (T) column has a single coefficient c multiplies its
+// node's multiplier by c (codegen/gen.py:614-621), base-case products and
+// peeling products use the node's multiplier as alpha (linalg/linalg.hpp:266,
+// 201-244).  beta reaches only the root's W stage and peeling; children use
+// 0 (algorithms/strassen.hpp:459).  With num_steps = 0 the root is itself
+// the base case, MatMul(A, B, C) with beta 0 (algorithms/strassen.hpp:385-388).
+//
+// Addresses are (role, offset) pairs resolved at launch time, so a plan is
+// built and uploaded once per shape and replayed for any buffers.
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace fmplan {
+
+namespace {
+thread_local std::string t_err;
+const int64_t kLdAlign = 16;    // elements; 128 B for FP64
+const int64_t kSlabAlign = 64;  // elements; keeps every buffer 256 B aligned
+
+struct Node {
+  FmOp a, b, c;
+  double mu;
+};
+
+FmOp op_make(int role, int64_t off, int64_t ld) {
+  FmOp o;
+  o.role = role;
+  o.pad = 0;
+  o.off = off;
+  o.ld = ld;
+  return o;
+}
+
+FmOp op_sub(const FmOp& o, int64_t i, int64_t j) { return op_make(o.role, o.off + i + j * o.ld, o.ld); }
+
+FmGemmDesc gdesc(FmOp a, FmOp b, FmOp c, int64_t M, int64_t N, int64_t K, double alpha,
+                 double beta) {
+  FmGemmDesc d;
+  memset(&d, 0, sizeof(d));
+  d.a = a;
+  d.b = b;
+  d.c = c;
+  d.M = (int32_t)M;
+  d.N = (int32_t)N;
+  d.K = (int32_t)K;
+  d.alpha = alpha;
+  d.beta = beta;
+  return d;
+}
+
+Launch stage_launch(fm_dtype dt, LaunchKind kind, const FmAlgDesc* alg, std::vector<FmOp> ops,
+                    int64_t rows, int64_t cols, int64_t out_off, int64_t out_ld, int64_t out_slab,
+                    double x, double beta) {
+  Launch L;
+  L.kind = kind;
+  L.alg = alg;
+  L.stage = kind == LK_S ? FM_STAGE_S : kind == LK_T ? FM_STAGE_T : FM_STAGE_W;
+  L.st.rows = (int32_t)rows;
+  L.st.cols = (int32_t)cols;
+  L.st.nodes = (int32_t)ops.size();
+  L.st.out_off = out_off;
+  L.st.out_ld = out_ld;
+  L.st.out_slab = out_slab;
+  L.st.x = x;
+  L.st.beta = beta;
+  const int vmax = dt == FM_F64 ? 2 : 4;
+  bool ok = rows % vmax == 0 && out_ld % vmax == 0 && out_slab % vmax == 0 && out_off % vmax == 0;
+  for (const FmOp& o : ops) ok = ok && o.off % vmax == 0 && o.ld % vmax == 0;
+  L.vec_ok = ok ? vmax : 1;
+  L.ops = std::move(ops);
+  return L;
+}
+
+// Peeling products of one node (linalg/linalg.hpp:201-244).
+void peel_descs(const Node& nd, const Dims& D, int am, int ak, int an, double beta,
+                std::vector<FmGemmDesc>& out) {
+  const int64_t erA = D.m % am, ecA = D.k % ak, ecB = D.n % an;
+  auto mk = [&](FmOp a, FmOp b, FmOp c, int64_t M, int64_t N, int64_t K, double bta) {
+    if (M > 0 && N > 0) out.push_back(gdesc(a, b, c, M, N, K, nd.mu, bta));
+  };
+  if (ecA > 0) {
+    const int64_t row_dim = D.m - erA, col_dim = D.n - ecB, kk = D.k - ecA;
+    mk(op_sub(nd.a, 0, kk), op_sub(nd.b, kk, 0), nd.c, row_dim, col_dim, ecA, 1.0);
+  }
+  if (ecB > 0) {
+    const int64_t s = D.n - ecB;
+    mk(nd.a, op_sub(nd.b, 0, s), op_sub(nd.c, 0, s), D.m, ecB, D.k, beta);
+  }
+  if (erA > 0) {
+    const int64_t s = D.m - erA, nc = D.n - ecB;
+    mk(op_sub(nd.a, s, 0), nd.b, op_sub(nd.c, s, 0), erA, nc, D.k, beta);
+  }
+}
+
+Launch gemm_launch(std::vector<FmGemmDesc> descs, int cfg, int stage, fm_dtype dt) {
+  int bm, bn;
+  fm_tile_dims(cfg, &bm, &bn);
+  int32_t t = 0;
+  bool uniform = true;
+  for (FmGemmDesc& d : descs) {
+    d.tiles_m = (int32_t)((d.M + bm - 1) / bm);
+    d.tiles_n = (int32_t)((d.N + bn - 1) / bn);
+    d.tile_begin = t;
+    t += d.tiles_m * d.tiles_n;
+    uniform = uniform && d.M == descs[0].M && d.N == descs[0].N;
+  }
+  Launch L;
+  L.kind = LK_GEMM;
+  L.stage = stage;
+  L.gm.ndesc = (int32_t)descs.size();
+  L.gm.total_tiles = t;
+  L.gm.uniform = uniform ? 1 : 0;
+  // 16-byte loads need every A/B column start aligned: even (FP64) or
+  // multiple-of-4 (FP32) offsets and leading dimensions; bases are checked
+  // at launch
+  const int vl = dt == FM_F64 ? 2 : 4;
+  bool vec = true;
+  for (const FmGemmDesc& d : descs)
+    vec = vec && d.a.off % vl == 0 && d.a.ld % vl == 0 && d.b.off % vl == 0 && d.b.ld % vl == 0;
+  L.vec_ok = vec ? 1 : 0;
+  L.cfg = cfg;
+  L.gdescs = std::move(descs);
+  return L;
+}
+
+// Leaf products on int8 digits: descriptors plus digit-plane workspace,
+// appended at byte offset *ws_bytes (advanced).
+fm_status ozaki_launch(int S, const std::vector<FmGemmDesc>& prods, size_t* ws_bytes, Launch& L) {
+  int32_t t = 0;
+  int max_rows = 0, max_cols = 0;
+  size_t off = (size_t)align_up((int64_t)*ws_bytes, 256);
+  for (const FmGemmDesc& g : prods) {
+    FmOzakiDesc d;
+    memset(&d, 0, sizeof(d));
+    d.a = g.a;
+    d.b = g.b;
+    d.c = g.c;
+    d.M = g.M;
+    d.N = g.N;
+    d.K = g.K;
+    d.Kp = (int32_t)align_up(g.K, 64);
+    if (d.Kp > fm_ozaki_max_kp(S))
+      return set_err(FM_ERR_SHAPE, "ozaki_int8: leaf K too large for exact int32 sums");
+    d.rows_pad = (int32_t)align_up(g.M, 64);
+    d.cols_pad = (int32_t)align_up(g.N, 64);
+    d.tiles_m = d.rows_pad / 64;
+    d.tiles_n = d.cols_pad / 64;
+    d.tile_begin = t;
+    t += d.tiles_m * d.tiles_n;
+    d.alpha = g.alpha;
+    d.beta = g.beta;
+    d.a_slc = (int64_t)off;
+    off += (size_t)S * d.rows_pad * d.Kp;
+    d.b_slc = (int64_t)off;
+    off += (size_t)S * d.cols_pad * d.Kp;
+    d.a_exp = (int64_t)off;
+    off += align_up(4 * (int64_t)d.rows_pad, 256);
+    d.b_exp = (int64_t)off;
+    off += align_up(4 * (int64_t)d.cols_pad, 256);
+    max_rows = std::max(max_rows, d.rows_pad);
+    max_cols = std::max(max_cols, d.cols_pad);
+    L.odescs.push_back(d);
+  }
+  L.kind = LK_OZAKI;
+  L.stage = FM_STAGE_LEAF;
+  L.oz.ndesc = (int32_t)L.odescs.size();
+  L.oz.total_tiles = t;
+  L.oz_slices = S;
+  L.oz_rows = max_rows;
+  L.oz_cols = max_cols;
+  *ws_bytes = off;
+  return FM_OK;
+}
+
+}  // namespace
+
+int g_tile_override = -1;
+int g_ozaki_slices = 8;
+
+fm_status set_err(fm_status s, const std::string& msg) {
+  t_err = msg;
+  return s;
+}
+fm_status hip_err(hipError_t e, const char* what) {
+  return set_err(FM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+const std::string& last_error() { return t_err; }
+void clear_error() { t_err.clear(); }
+
+int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+double round_scalar(double v, fm_dtype dt) { return dt == FM_F32 ? (double)(float)v : v; }
+int elem_size(fm_dtype dt) { return dt == FM_F64 ? 8 : 4; }
+
+// Tile of a grouped launch: smallest estimated time over the configurations,
+// where a configuration's throughput per tile (eff, measured on MI355X at
+// 8192^3) and its resident work groups per CU set how many rounds of tiles
+// the launch takes.
+int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods) {
+  if (g_tile_override >= 0) return g_tile_override;
+  struct C {
+    int cfg;
+    double eff;
+    int per_cu;
+  };
+  static const C f64c[] = {{FM_TILE_256x128_W8, 1.00, 1}, {FM_TILE_128x256_W8, 0.99, 1},
+                           {FM_TILE_128x128_W16, 0.93, 2}, {FM_TILE_128x128_W8, 0.90, 2},
+                           {FM_TILE_128x64_W8, 0.91, 3},  {FM_TILE_64x64, 0.80, 4}};
+  static const C f32c[] = {{FM_TILE_256x256_W16, 1.00, 1}, {FM_TILE_256x128_W8, 0.90, 2},
+                           {FM_TILE_128x256_W8, 0.90, 2},  {FM_TILE_128x128_W8, 0.82, 4},
+                           {FM_TILE_64x128, 0.77, 4},      {FM_TILE_128x64, 0.74, 4},
+                           {FM_TILE_64x64, 0.69, 8}};
+  static const C splitc[] = {{FM_TILE_128x128, 1.00, 2}, {FM_TILE_64x64, 0.80, 4}};
+  const C* cands = mode == FM_MODE_SPLIT_BF16 ? splitc : dt == FM_F64 ? f64c : f32c;
+  const int nc = mode == FM_MODE_SPLIT_BF16 ? 2 : dt == FM_F64 ? 6 : 7;
+  int best = FM_TILE_64x64;
+  double best_t = 1e300;
+  for (int i = 0; i < nc; ++i) {
+    int bm, bn;
+    fm_tile_dims(cands[i].cfg, &bm, &bn);
+    int64_t tiles = 0;
+    for (const FmGemmDesc& d : prods) tiles += ((d.M + bm - 1) / bm) * ((d.N + bn - 1) / bn);
+    const int64_t slots = 256LL * cands[i].per_cu;
+    const double rounds = (double)((tiles + slots - 1) / slots);
+    const double t = rounds * bm * bn * cands[i].per_cu / cands[i].eff;
+    if (t < best_t * 0.999) {
+      best_t = t;
+      best = cands[i].cfg;
+    }
+  }
+  return best;
+}
+
+fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan& out) {
+  out = SubPlan();
+  const FmOp A0 = op_make(p.role_a, 0, p.lda), B0 = op_make(p.role_b, 0, p.ldb),
+             C0 = op_make(p.role_c, 0, p.ldc);
+  if (!p.alg) {  // classical product, beta honoured (MatMul, linalg/linalg.hpp:261-269)
+    out.leaves.push_back(gdesc(A0, B0, C0, p.m, p.n, p.k, round_scalar(p.alpha, dt),
+                               round_scalar(p.beta, dt)));
+    out.leaf = {p.m, p.k, p.n};
+    return FM_OK;
+  }
+  const FmAlgDesc* alg = p.alg;
+  const int L = p.steps;
+  std::vector<Dims> dims(L + 1);
+  dims[0] = {p.m, p.k, p.n};
+  for (int l = 0; l < L; ++l) {
+    dims[l + 1] = {dims[l].m / alg->m, dims[l].k / alg->k, dims[l].n / alg->n};
+    if (dims[l + 1].m <= 0 || dims[l + 1].k <= 0 || dims[l + 1].n <= 0)
+      return set_err(FM_ERR_SHAPE, "num_steps too deep for this shape: a block dimension "
+                                   "reaches 0 at level " + std::to_string(l + 1));
+  }
+  std::vector<std::vector<Node>> lv(L + 1);
+  lv[0].push_back(Node{A0, B0, C0, round_scalar(p.alpha, dt)});
+  std::vector<int64_t> m_off(L), m_ld(L), m_slab(L);
+  int64_t ws = ws_base;
+  const int R = alg->R, NS = alg->ns, NT = alg->nt;
+  for (int l = 0; l < L; ++l) {
+    const Dims c = dims[l + 1];
+    const int64_t Q = (int64_t)lv[l].size();
+    const int64_t s_ld = align_up(c.m, kLdAlign), s_slab = align_up(s_ld * c.k, kSlabAlign);
+    const int64_t t_ld = align_up(c.k, kLdAlign), t_slab = align_up(t_ld * c.n, kSlabAlign);
+    m_ld[l] = align_up(c.m, kLdAlign);
+    m_slab[l] = align_up(m_ld[l] * c.n, kSlabAlign);
+    const int64_t s_off = ws;
+    ws += Q * NS * s_slab;
+    const int64_t t_off = ws;
+    ws += Q * NT * t_slab;
+    m_off[l] = ws;
+    ws += Q * R * m_slab[l];
+    lv[l + 1].reserve(Q * R);
+    for (int64_t q = 0; q < Q; ++q) {
+      const Node& nd = lv[l][q];
+      for (int r = 0; r < R; ++r) {
+        Node ch;
+        if (alg->a_kind[r])
+          ch.a = op_make(FM_ROLE_WS, s_off + (q * NS + alg->a_index[r]) * s_slab, s_ld);
+        else
+          ch.a = op_sub(nd.a, (alg->a_index[r] / alg->k) * c.m, (alg->a_index[r] % alg->k) * c.k);
+        if (alg->b_kind[r])
+          ch.b = op_make(FM_ROLE_WS, t_off + (q * NT + alg->b_index[r]) * t_slab, t_ld);
+        else
+          ch.b = op_sub(nd.b, (alg->b_index[r] / alg->n) * c.k, (alg->b_index[r] % alg->n) * c.n);
+        ch.c = op_make(FM_ROLE_WS, m_off[l] + (q * R + r) * m_slab[l], m_ld[l]);
+        double mu = round_scalar(nd.mu * round_scalar(alg->a_coef(r, p.x), dt), dt);
+        ch.mu = round_scalar(mu * round_scalar(alg->b_coef(r, p.x), dt), dt);
+        lv[l + 1].push_back(ch);
+      }
+    }
+    if (NS > 0) {
+      std::vector<FmOp> ops;
+      for (const Node& nd : lv[l]) ops.push_back(nd.a);
+      out.pre.push_back(stage_launch(dt, LK_S, alg, std::move(ops), c.m, c.k, s_off, s_ld, s_slab,
+                                     p.x, 0.0));
+    }
+    if (NT > 0) {
+      std::vector<FmOp> ops;
+      for (const Node& nd : lv[l]) ops.push_back(nd.b);
+      out.pre.push_back(stage_launch(dt, LK_T, alg, std::move(ops), c.k, c.n, t_off, t_ld, t_slab,
+                                     p.x, 0.0));
+    }
+  }
+  const Dims d = dims[L];
+  for (const Node& nd : lv[L]) out.leaves.push_back(gdesc(nd.a, nd.b, nd.c, d.m, d.n, d.k, nd.mu, 0.0));
+  out.leaf = d;
+  out.w.resize(L);
+  out.peels.resize(L);
+  for (int l = L - 1; l >= 0; --l) {
+    const Dims c = dims[l + 1];
+    const double bl = l == 0 ? round_scalar(p.beta, dt) : 0.0;
+    std::vector<FmOp> ops;
+    for (const Node& nd : lv[l]) ops.push_back(nd.c);
+    out.w[l] = stage_launch(dt, LK_W, alg, std::move(ops), c.m, c.n, m_off[l], m_ld[l], m_slab[l],
+                            p.x, bl);
+    for (const Node& nd : lv[l]) peel_descs(nd, dims[l], alg->m, alg->k, alg->n, bl, out.peels[l]);
+  }
+  out.depth = L;
+  out.ws_elems = ws - ws_base;
+  return FM_OK;
+}
+
+fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int persistent_grid) {
+  for (SubPlan& s : subs)
+    for (Launch& L : s.pre) P.launches.push_back(std::move(L));
+  std::vector<FmGemmDesc> leaves;
+  for (SubPlan& s : subs) leaves.insert(leaves.end(), s.leaves.begin(), s.leaves.end());
+  size_t wsb = (size_t)ws_elems * elem_size(P.dtype);
+  if (!leaves.empty()) {
+    if (P.mode == FM_MODE_OZAKI_INT8) {
+      Launch L;
+      fm_status st = ozaki_launch(P.slices, leaves, &wsb, L);
+      if (st != FM_OK) return st;
+      P.launches.push_back(std::move(L));
+    } else {
+      const int cfg = pick_tile(P.dtype, P.mode, leaves);
+      P.launches.push_back(gemm_launch(std::move(leaves), cfg, FM_STAGE_LEAF, P.dtype));
+      P.launches.back().gm.grid = persistent_grid;
+    }
+  }
+  int maxd = 0;
+  for (const SubPlan& s : subs) maxd = std::max(maxd, s.depth);
+  for (int l = maxd - 1; l >= 0; --l) {
+    std::vector<FmGemmDesc> peel;
+    for (SubPlan& s : subs) {
+      if (s.depth <= l) continue;
+      P.launches.push_back(std::move(s.w[l]));
+      peel.insert(peel.end(), s.peels[l].begin(), s.peels[l].end());
+    }
+    // peeling products stay on the plain kernel in OZAKI mode (FP64) and
+    // take the split kernel in SPLIT_BF16 mode
+    if (!peel.empty()) P.launches.push_back(gemm_launch(std::move(peel), FM_TILE_64x64, FM_STAGE_PEEL, P.dtype));
+  }
+  P.ws_bytes = wsb;
+  return FM_OK;
+}
+
+Plan::~Plan() {
+  if (dbuf) (void)hipFree(dbuf);
+}
+
+fm_status Plan::upload() {
+  std::vector<char> buf;
+  auto put = [&](const void* src, size_t bytes) {
+    const size_t off = (size_t)align_up((int64_t)buf.size(), 256);
+    buf.resize(off + bytes);
+    if (bytes) memcpy(buf.data() + off, src, bytes);
+    return off;
+  };
+  for (Launch& L : launches) {
+    if (L.kind == LK_GEMM)
+      L.dev_off = put(L.gdescs.data(), L.gdescs.size() * sizeof(FmGemmDesc));
+    else if (L.kind == LK_OZAKI)
+      L.dev_off = put(L.odescs.data(), L.odescs.size() * sizeof(FmOzakiDesc));
+    else
+      L.dev_off = put(L.ops.data(), L.ops.size() * sizeof(FmOp));
+  }
+  ext_off = put(ext.data(), ext.size() * sizeof(void*));
+  if (buf.empty()) return FM_OK;
+  hipError_t e = hipMalloc(&dbuf, buf.size());
+  if (e != hipSuccess) {
+    dbuf = nullptr;
+    return hip_err(e, "hipMalloc(plan)");
+  }
+  e = hipMemcpy(dbuf, buf.data(), buf.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_err(e, "hipMemcpy(plan)");
+  for (Launch& L : launches) {
+    char* p = (char*)dbuf + L.dev_off;
+    if (L.kind == LK_GEMM)
+      L.gm.descs = reinterpret_cast<const FmGemmDesc*>(p);
+    else if (L.kind == LK_OZAKI)
+      L.oz.descs = reinterpret_cast<const FmOzakiDesc*>(p);
+    else
+      L.st.ops = reinterpret_cast<const FmOp*>(p);
+  }
+  return FM_OK;
+}
+
+// ---- profiling ---------------------------------------------------------------
+namespace {
+struct ProfRec {
+  int stage;
+  hipEvent_t e0, e1;
+};
+std::vector<ProfRec> g_prof_pending;
+std::vector<hipEvent_t> g_event_pool;
+fm_profile g_prof_acc;
+
+hipEvent_t pool_event() {
+  if (!g_event_pool.empty()) {
+    hipEvent_t e = g_event_pool.back();
+    g_event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+}  // namespace
+
+bool g_prof_on = false;
+
+fm_status profile_read(fm_profile* out) {
+  fm_status st = FM_OK;
+  for (ProfRec& r : g_prof_pending) {
+    float ms = 0.f;
+    hipError_t e = hipEventSynchronize(r.e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, r.e0, r.e1);
+    if (e != hipSuccess && st == FM_OK) st = hip_err(e, "profile event");
+    g_prof_acc.ms[r.stage] += ms;
+    g_prof_acc.launches[r.stage] += 1;
+    g_event_pool.push_back(r.e0);
+    g_event_pool.push_back(r.e1);
+  }
+  g_prof_pending.clear();
+  if (out) *out = g_prof_acc;
+  memset(&g_prof_acc, 0, sizeof(g_prof_acc));
+  return st;
+}
+
+fm_status Plan::run(const void* A, const void* B, void* C, void* ws, hipStream_t s) const {
+  FmBases bases;
+  bases.p[FM_ROLE_A] = const_cast<void*>(A);
+  bases.p[FM_ROLE_B] = const_cast<void*>(B);
+  bases.p[FM_ROLE_C] = C;
+  bases.p[FM_ROLE_WS] = ws;
+  bases.ext = ext.empty() ? nullptr : reinterpret_cast<void* const*>((char*)dbuf + ext_off);
+  bool aligned = true;
+  for (int i = 0; i < 4; ++i) aligned = aligned && ((uintptr_t)bases.p[i] % 16 == 0);
+  for (void* p : ext) aligned = aligned && ((uintptr_t)p % 16 == 0);
+  for (const Launch& L : launches) {
+    hipError_t e = hipSuccess;
+    ProfRec pr{L.stage, nullptr, nullptr};
+    if (g_prof_on) {
+      pr.e0 = pool_event();
+      pr.e1 = pool_event();
+      if (pr.e0) (void)hipEventRecord(pr.e0, s);
+    }
+    if (L.kind == LK_OZAKI) {
+      FmOzakiArgs o = L.oz;
+      o.bases = bases;
+      e = fm_ozaki_launch(L.oz_slices, o, L.oz_rows, L.oz_cols, s);
+    } else if (L.kind == LK_GEMM) {
+      FmGemmArgs g = L.gm;
+      g.bases = bases;
+      g.vec = (L.vec_ok && aligned) ? 1 : 0;
+      if (mode == FM_MODE_SPLIT_BF16)
+        e = fm_gemm_launch_bf16x3(L.cfg, g, s);
+      else
+        e = dtype == FM_F64 ? fm_gemm_launch_f64(L.cfg, g, s) : fm_gemm_launch_f32(L.cfg, g, s);
+    } else {
+      FmStageArgs a = L.st;
+      a.bases = bases;
+      a.vec = aligned ? L.vec_ok : 1;
+      const FmAlgDesc* g = L.alg;
+      FmStageLaunch fn;
+      if (dtype == FM_F64)
+        fn = L.kind == LK_S ? g->s_d : L.kind == LK_T ? g->t_d : g->w_d;
+      else
+        fn = L.kind == LK_S ? g->s_f : L.kind == LK_T ? g->t_f : g->w_f;
+      e = fn(a, s);
+    }
+    if (g_prof_on && pr.e0 && pr.e1) {
+      (void)hipEventRecord(pr.e1, s);
+      g_prof_pending.push_back(pr);
+    }
+    if (e != hipSuccess) return hip_err(e, "kernel launch");
+  }
+  return FM_OK;
+}
+
+}  // namespace fmplan

@@ -1,0 +1,105 @@
+// plan.hpp -- execution plans: the reference's recursion unrolled into
+// batched launches over all nodes of each level.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/fastmm.h"
+#include "alg_desc.hpp"
+#include "gemm_mfma.hpp"
+#include "gemm_ozaki.hpp"
+
+namespace fmplan {
+
+fm_status set_err(fm_status s, const std::string& msg);
+fm_status hip_err(hipError_t e, const char* what);
+const std::string& last_error();
+void clear_error();
+
+enum LaunchKind { LK_S, LK_T, LK_W, LK_GEMM, LK_OZAKI };
+
+// One kernel launch.  Descriptors stay on the host until Plan::upload packs
+// them into the plan's device buffer.
+struct Launch {
+  LaunchKind kind = LK_GEMM;
+  int stage = 0;                   // fm_stage_kind, for profiling
+  const FmAlgDesc* alg = nullptr;  // stage launches: whose combine functions
+  FmStageArgs st{};
+  FmGemmArgs gm{};
+  FmOzakiArgs oz{};
+  int cfg = 0;
+  int vec_ok = 1;
+  int oz_slices = 0, oz_rows = 0, oz_cols = 0;
+  std::vector<FmOp> ops;
+  std::vector<FmGemmDesc> gdescs;
+  std::vector<FmOzakiDesc> odescs;
+  size_t dev_off = 0;
+};
+
+struct Plan {
+  fm_dtype dtype = FM_F64;
+  fm_mode mode = FM_MODE_PLAIN;
+  int slices = 0;
+  std::vector<Launch> launches;
+  std::vector<void*> ext;          // role table (roles >= 4) of batched plans
+  void* dbuf = nullptr;            // descriptors + role table on the device
+  size_t ext_off = 0;
+  size_t ws_bytes = 0;
+  ~Plan();
+  fm_status upload();
+  // Launch every kernel on `s`; A/B/C/ws fill roles 0-3.
+  fm_status run(const void* A, const void* B, void* C, void* ws, hipStream_t s) const;
+};
+
+// One multiply C = alpha op(A) op(B) + beta C, as the reference's recursion.
+struct Problem {
+  const FmAlgDesc* alg = nullptr;  // nullptr: classical product (numsteps ignored)
+  int steps = 0;
+  int64_t m = 0, k = 0, n = 0;
+  int64_t lda = 0, ldb = 0, ldc = 0;
+  int role_a = FM_ROLE_A, role_b = FM_ROLE_B, role_c = FM_ROLE_C;
+  double x = 1e-8, alpha = 1.0, beta = 0.0;
+};
+
+struct Dims {
+  int64_t m, k, n;
+};
+
+// The pieces of one problem's plan, before launches of several problems
+// are merged.
+struct SubPlan {
+  std::vector<Launch> pre;                          // S/T stages, level order
+  std::vector<FmGemmDesc> leaves;                   // base-case products
+  std::vector<Launch> w;                            // W stage of level l at w[l]
+  std::vector<std::vector<FmGemmDesc>> peels;       // peeling products of level l
+  int depth = 0;
+  int64_t ws_elems = 0;
+  Dims leaf{0, 0, 0};
+};
+
+// Build the sub-plan of `p`, placing its workspace at element offset ws_base.
+fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan& out);
+
+// Merge sub-plans into one launch sequence:
+//   all S/T stages; one grouped leaf launch; per level (deepest first) the W
+//   stages, then one grouped launch of that level's peeling products.
+// persistent_grid > 0 runs the grouped leaf launch with that many work
+// groups, each looping over tiles.
+fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int persistent_grid = 0);
+
+int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods);
+extern int g_tile_override;
+extern int g_ozaki_slices;
+
+// profiling (fm_profile_enable / fm_profile_read)
+extern bool g_prof_on;
+fm_status profile_read(fm_profile* out);
+
+int elem_size(fm_dtype dt);
+double round_scalar(double v, fm_dtype dt);
+int64_t align_up(int64_t v, int64_t a);
+
+}  // namespace fmplan

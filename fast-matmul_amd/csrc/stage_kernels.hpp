@@ -71,7 +71,7 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage(FmStageArgs a) {
   constexpr int NOUT_ARR = NOUT > 0 ? NOUT : 1;
   const int q = blockIdx.y;
   const FmOp op = a.ops[q];
-  const T* src = reinterpret_cast<const T*>(a.bases.p[op.role]) + op.off;
+  const T* src = reinterpret_cast<const T*>(fm_base(a.bases, op.role)) + op.off;
   T* dst = reinterpret_cast<T*>(a.bases.p[FM_ROLE_WS]) + a.out_off +
            (int64_t)q * NOUT * a.out_slab;
   const uint32_t rv = (uint32_t)a.rows / VEC;
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage(FmStageArgs a) {
   constexpr int MN = Cmb::m * Cmb::n;
   const int q = blockIdx.y;
   const FmOp op = a.ops[q];
-  T* c = reinterpret_cast<T*>(a.bases.p[op.role]) + op.off;
+  T* c = reinterpret_cast<T*>(fm_base(a.bases, op.role)) + op.off;
   const T* mb = reinterpret_cast<const T*>(a.bases.p[FM_ROLE_WS]) + a.out_off +
                 (int64_t)q * R * a.out_slab;
   const uint32_t rv = (uint32_t)a.rows / VEC;

@@ -35,12 +35,19 @@ def algorithm_names():
     return [LIB.fm_algorithm_name(i).decode() for i in range(LIB.fm_num_algorithms())]
 
 
+def _is_col_major(t: torch.Tensor) -> bool:
+    m, n = t.shape
+    if m <= 1:  # one row: any column stride >= 1 is a valid leading dimension
+        return n <= 1 or t.stride(1) >= 1
+    if n <= 1:  # one column: rows must be contiguous
+        return t.stride(0) == 1
+    return t.stride(0) == 1 and t.stride(1) >= m
+
+
 def _col_major(t: torch.Tensor, name: str) -> torch.Tensor:
     if t.dim() != 2:
         raise ValueError("%s must be 2-D" % name)
-    if t.stride(0) == 1 and t.stride(1) >= max(1, t.shape[0]):
-        return t
-    if t.shape[1] == 1 and t.stride(0) == 1:
+    if _is_col_major(t):
         return t
     raise ValueError("%s must be a column-major view (stride(0) == 1); use "
                      "fastmm.col_major(%s)" % (name, name))
@@ -48,7 +55,7 @@ def _col_major(t: torch.Tensor, name: str) -> torch.Tensor:
 
 def col_major(t: torch.Tensor) -> torch.Tensor:
     """Column-major copy of a 2-D tensor (no copy if it already is one)."""
-    if t.stride(0) == 1 and t.stride(1) >= max(1, t.shape[0]):
+    if _is_col_major(t):
         return t
     return t.t().contiguous().t()
 
@@ -59,7 +66,10 @@ def empty(m: int, n: int, dtype=torch.float64, device="cuda") -> torch.Tensor:
 
 
 def _ld(t: torch.Tensor) -> int:
-    return max(1, t.stride(1) if t.shape[1] > 1 else max(t.shape[0], 1))
+    m, n = t.shape
+    if n > 1:
+        return max(1, m, t.stride(1)) if m > 1 else max(1, t.stride(1))
+    return max(1, m)
 
 
 def _stream_handle(t: torch.Tensor):
@@ -138,6 +148,11 @@ def set_gemm_tile(cfg: int = -1):
     check(LIB.fm_set_gemm_tile(int(cfg)))
 
 
+def set_ozaki_slices(slices: int = 8):
+    """Digit count of the ozaki_int8 leaf mode (6, 7 or 8)."""
+    check(LIB.fm_set_ozaki_slices(int(slices)))
+
+
 def profile_enable(on: bool = True):
     """Bracket every library launch with HIP events on its stream."""
     LIB.fm_profile_enable(1 if on else 0)
@@ -148,6 +163,79 @@ def profile_read():
     p = _lib.FmProfile()
     check(LIB.fm_profile_read(ctypes.byref(p)))
     return {nm: (p.ms[i], p.launches[i]) for i, nm in enumerate(_lib.STAGES)}
+
+
+class Batch:
+    """Many independent C_i := alpha_i A_i B_i + beta_i C_i in one launch
+    sequence (one grouped launch of all base-case products), optionally
+    replayed from a captured hipGraph.  ``problems`` is a list of dicts with
+    keys A, B, C (column-major GPU tensors of one dtype) and optional alg,
+    num_steps, alpha, beta; a missing alg / num_steps is autotuned."""
+
+    def __init__(self, problems, mode: str = "plain", candidates=None, max_steps: int = 2,
+                 graph: bool = True, workspace: Optional[torch.Tensor] = None):
+        if not problems:
+            raise ValueError("empty batch")
+        self._keep = []
+        arr = (_lib.FmProblem * len(problems))()
+        dtype = None
+        for i, p in enumerate(problems):
+            A, B, C = _col_major(p["A"], "A"), _col_major(p["B"], "B"), _col_major(p["C"], "C")
+            dtype = dtype or A.dtype
+            if A.dtype != dtype or B.dtype != dtype or C.dtype != dtype:
+                raise TypeError("all matrices of a batch share one dtype")
+            self._keep += [A, B, C]
+            alg = p.get("alg")
+            arr[i] = _lib.FmProblem(alg.encode() if alg else None, int(p.get("num_steps", -1)),
+                                    A.shape[0], A.shape[1], B.shape[1], A.data_ptr(), _ld(A),
+                                    B.data_ptr(), _ld(B), C.data_ptr(), _ld(C),
+                                    float(p.get("alpha", 1.0)), float(p.get("beta", 0.0)))
+        cands = None
+        ncand = 0
+        if candidates:
+            cands = (ctypes.c_char_p * len(candidates))(*[c.encode() for c in candidates])
+            ncand = len(candidates)
+        ws_ptr, ws_bytes = None, 0
+        if workspace is not None:
+            ws_ptr = ctypes.c_void_p(workspace.data_ptr())
+            ws_bytes = workspace.numel() * workspace.element_size()
+            self._keep.append(workspace)
+        h = ctypes.c_void_p()
+        check(LIB.fm_batch_create(_DT[dtype], _lib.MODES[mode], arr, len(problems), cands, ncand,
+                                  max_steps, 1 if graph else 0, ws_ptr, ws_bytes, ctypes.byref(h)))
+        self._h = h
+        self.count = len(problems)
+        self._device = self._keep[0].device
+
+    def run(self, timed: bool = False):
+        ms = ctypes.c_float(0.0)
+        s = ctypes.c_void_p(torch.cuda.current_stream(self._device).cuda_stream)
+        check(LIB.fm_batch_run(self._h, s, ctypes.byref(ms) if timed else None))
+        return ms.value if timed else None
+
+    def choice(self, i: int):
+        a = ctypes.c_char_p()
+        st = ctypes.c_int()
+        check(LIB.fm_batch_choice(self._h, i, ctypes.byref(a), ctypes.byref(st)))
+        return a.value.decode(), st.value
+
+    def workspace_bytes(self) -> int:
+        return LIB.fm_batch_workspace_bytes(self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            LIB.fm_batch_destroy(h)
+            self._h = None
+
+
+def autotune_estimate(alg: Optional[str], m: int, k: int, n: int, num_steps: int,
+                      dtype=torch.float32, mode: str = "plain") -> float:
+    """The autotuner's cost-model estimate in microseconds."""
+    us = ctypes.c_double(0.0)
+    check(LIB.fm_autotune_estimate(_DT[dtype], _lib.MODES[mode], alg.encode() if alg else None,
+                                   m, k, n, num_steps, ctypes.byref(us)))
+    return us.value
 
 
 # ---------------------------------------------------------------------------

@@ -29,6 +29,8 @@ EXPORTS = (
     "fm_fastmatmul", "fm_workspace_query", "fm_gemm", "fm_num_algorithms",
     "fm_algorithm_name", "fm_algorithm_info", "fm_status_string", "fm_last_error",
     "fm_release_caches", "fm_profile_enable", "fm_profile_read", "fm_set_gemm_tile",
+    "fm_set_ozaki_slices", "fm_batch_create", "fm_batch_run", "fm_batch_choice",
+    "fm_batch_workspace_bytes", "fm_batch_destroy", "fm_autotune_estimate",
 )
 
 STAGES = ("S", "T", "W", "LEAF", "PEEL")
@@ -36,6 +38,14 @@ STAGES = ("S", "T", "W", "LEAF", "PEEL")
 
 class FmProfile(ctypes.Structure):
     _fields_ = [("ms", ctypes.c_double * 5), ("launches", ctypes.c_int64 * 5)]
+
+
+class FmProblem(ctypes.Structure):
+    _fields_ = [("alg", ctypes.c_char_p), ("num_steps", ctypes.c_int), ("m", ctypes.c_int),
+                ("k", ctypes.c_int), ("n", ctypes.c_int), ("A", ctypes.c_void_p),
+                ("lda", ctypes.c_int64), ("B", ctypes.c_void_p), ("ldb", ctypes.c_int64),
+                ("C", ctypes.c_void_p), ("ldc", ctypes.c_int64), ("alpha", ctypes.c_double),
+                ("beta", ctypes.c_double)]
 
 
 class FastMMError(RuntimeError):
@@ -77,6 +87,24 @@ def _load():
     lib.fm_profile_read.argtypes = [ctypes.POINTER(FmProfile)]
     lib.fm_set_gemm_tile.restype = c_int
     lib.fm_set_gemm_tile.argtypes = [c_int]
+    lib.fm_set_ozaki_slices.restype = c_int
+    lib.fm_set_ozaki_slices.argtypes = [c_int]
+    lib.fm_batch_create.restype = c_int
+    lib.fm_batch_create.argtypes = [c_int, c_int, ctypes.POINTER(FmProblem), c_int,
+                                    ctypes.POINTER(ctypes.c_char_p), c_int, c_int, c_int, vp,
+                                    ctypes.c_size_t, ctypes.POINTER(vp)]
+    lib.fm_batch_run.restype = c_int
+    lib.fm_batch_run.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_float)]
+    lib.fm_batch_choice.restype = c_int
+    lib.fm_batch_choice.argtypes = [vp, c_int, ctypes.POINTER(ctypes.c_char_p),
+                                    ctypes.POINTER(c_int)]
+    lib.fm_batch_workspace_bytes.restype = ctypes.c_size_t
+    lib.fm_batch_workspace_bytes.argtypes = [vp]
+    lib.fm_batch_destroy.restype = None
+    lib.fm_batch_destroy.argtypes = [vp]
+    lib.fm_autotune_estimate.restype = c_int
+    lib.fm_autotune_estimate.argtypes = [c_int, c_int, ctypes.c_char_p, c_int, c_int, c_int,
+                                         c_int, ctypes.POINTER(c_double)]
     return lib
 
 

@@ -83,6 +83,47 @@ fm_status fm_gemm(fm_dtype dtype, fm_mode mode, int m, int k, int n, const void*
                   int64_t lda, const void* B, int64_t ldb, void* C, int64_t ldc, double alpha,
                   double beta, void* stream);
 
+/* ---- batches of independent multiplies -------------------------------------
+ * Many C_i := alpha_i A_i B_i + beta_i C_i in one launch sequence: every
+ * problem's S/T stages, ONE grouped launch of all base-case products, then
+ * the W stages and one grouped peeling launch per level.  A problem with
+ * alg == NULL (or num_steps < 0) gets its algorithm (from `candidates`,
+ * default {"strassen", "grey423_20_144", "smirnov333_23_139"}) and depth
+ * (0 .. max_steps, default 2) from the autotuner's cost model; depth 0 is
+ * the classical product with beta honoured.  With use_graph the launch
+ * sequence is captured once into a hipGraph and fm_batch_run replays it
+ * (the matrices' addresses are fixed at creation).  No reference
+ * counterpart: the reference multiplies one problem per call. */
+typedef struct {
+  const char* alg; /* NULL: autotuned */
+  int num_steps;   /* < 0: autotuned */
+  int m, k, n;
+  const void* A;
+  int64_t lda;
+  const void* B;
+  int64_t ldb;
+  void* C;
+  int64_t ldc;
+  double alpha, beta;
+} fm_problem;
+
+typedef struct fm_batch_s* fm_batch;
+
+fm_status fm_batch_create(fm_dtype dtype, fm_mode mode, const fm_problem* problems, int count,
+                          const char* const* candidates, int num_candidates, int max_steps,
+                          int use_graph, void* workspace, size_t workspace_bytes,
+                          fm_batch* out);
+fm_status fm_batch_run(fm_batch batch, void* stream, float* ms_out);
+/* algorithm ("classical" at depth 0) and depth chosen for problem `index` */
+fm_status fm_batch_choice(fm_batch batch, int index, const char** alg, int* num_steps);
+size_t fm_batch_workspace_bytes(fm_batch batch);
+void fm_batch_destroy(fm_batch batch);
+
+/* The autotuner's estimate (microseconds) for one multiply; alg == NULL or
+ * num_steps == 0 prices the classical product. */
+fm_status fm_autotune_estimate(fm_dtype dtype, fm_mode mode, const char* alg, int m, int k, int n,
+                               int num_steps, double* us);
+
 /* Registry (algorithms/all_algorithms.hpp:52-90, 247-326). */
 int fm_num_algorithms(void);
 const char* fm_algorithm_name(int index);
@@ -112,6 +153,11 @@ typedef struct {
 
 void fm_profile_enable(int on);
 fm_status fm_profile_read(fm_profile* out);
+
+/* Digit count S of FM_MODE_OZAKI_INT8 leaves (6, 7 or 8; default 8): each
+ * FP64 operand row/column becomes S int8 digits of 7 bits under a
+ * power-of-two scale, and S(S+1)/2 int8 products are summed. */
+fm_status fm_set_ozaki_slices(int slices);
 
 /* Tuning override: force leaf-GEMM tile configuration `cfg` (0 .. 10, see
  * FmTileCfg in the sources) for every plan built afterwards; -1 restores the

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     txt = open(os.path.join(ROOT, "include", "fastmm.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:const char\*|fm_status|int|void)\s+\**(fm_\w+)\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:const char\*|fm_status|int|void|size_t)\s+\**(fm_\w+)\(", txt, re.M)))
 
 
 def test_exports_every_declared_symbol():
@@ -91,3 +91,18 @@ def test_cpp_header_builds_against_the_library(tmp_path):
     spelling) compiles with a host compiler and links to libfastmm.so."""
     exe = _build_shim_example(tmp_path)
     assert os.path.exists(exe)
+
+
+def test_autotune_cost_model():
+    """The batch autotuner's cost model (host only): classical wins small
+    problems, a fast algorithm wins large ones, too-deep recursion is
+    infeasible."""
+    import math
+    import torch
+    import fastmm
+    f32 = torch.float32
+    assert fastmm.autotune_estimate(None, 512, 512, 512, 0, f32) < \
+        fastmm.autotune_estimate("strassen", 512, 512, 512, 2, f32)
+    assert fastmm.autotune_estimate("strassen", 8192, 8192, 8192, 1, f32) < \
+        fastmm.autotune_estimate(None, 8192, 8192, 8192, 0, f32)
+    assert math.isinf(fastmm.autotune_estimate("strassen", 3, 3, 3, 2, f32))

@@ -1,0 +1,84 @@
+"""Batched uneven multiplies (fm_batch_*): autotuned algorithm/depth per
+problem, one grouped leaf launch, optional hipGraph replay.
+
+Pins: integer inputs give every problem's exact product (bit-exact), for
+explicit and autotuned choices, through graph replays (run several times
+with host work in between, every replay checked); uniform(-1,1) FP32 inputs
+stay within FP32-level error of the FP64 product."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _problems(fm, shapes, dtype, dist, seed):
+    rng = np.random.default_rng(seed)
+    probs, host = [], []
+    for (m, k, n) in shapes:
+        if dist == "int":
+            A = rng.integers(-8, 9, (m, k)).astype(np.float64)
+            B = rng.integers(-8, 9, (k, n)).astype(np.float64)
+        else:
+            A = rng.random((m, k)) * 2 - 1
+            B = rng.random((k, n)) * 2 - 1
+        C0 = rng.integers(-8, 9, (m, n)).astype(np.float64)
+        t = lambda x: fm.col_major(torch.from_numpy(np.asfortranarray(x)).to(DEV, dtype))
+        probs.append({"A": t(A), "B": t(B), "C": t(C0)})
+        host.append((A, B, C0))
+    return probs, host
+
+
+def test_batch_explicit_and_autotuned_exact(fm):
+    shapes = [(576, 512, 640), (704, 1024, 512), (512, 576, 960), (1088, 512, 512),
+              (640, 640, 640), (512, 2048, 576)]
+    probs, host = _problems(fm, shapes, torch.float64, "int", 1)
+    probs[0].update(alg="strassen", num_steps=2)
+    probs[1].update(alg="grey423_20_144", num_steps=1, alpha=2.0, beta=-1.0)
+    probs[2].update(alg="smirnov333_23_139", num_steps=2)
+    probs[3].update(alg=None, num_steps=0, beta=0.5)  # depth 0: the classical product
+    b = fm.Batch(probs, graph=False)
+    b.run()
+    torch.cuda.synchronize()
+    for i, (p, (A, B, C0)) in enumerate(zip(probs, host)):
+        alpha, beta = p.get("alpha", 1.0), p.get("beta", 0.0)
+        alg, steps = b.choice(i)
+        want = alpha * (A @ B) + beta * C0
+        got = p["C"].cpu().numpy()
+        assert np.array_equal(got, want), (i, alg, steps)
+
+
+def test_batch_graph_replays(fm):
+    shapes = [(512 + 64 * i, 512 + 128 * (i % 3), 512 + 64 * (i % 5)) for i in range(12)]
+    probs, host = _problems(fm, shapes, torch.float64, "int", 2)
+    b = fm.Batch(probs, graph=True)
+    for rep in range(3):
+        for p in probs:
+            p["C"].fill_(7.0)
+        torch.cuda.synchronize()
+        b.run()
+        torch.cuda.synchronize()
+        _ = np.ones(1000).sum()  # host work between replays
+        for p, (A, B, C0) in zip(probs, host):
+            assert np.array_equal(p["C"].cpu().numpy(), A @ B)
+
+
+def test_batch_autotuner_choices_and_fp32_accuracy(fm, oracle_mod):
+    rng = np.random.default_rng(3)
+    vals = list(range(512, 4097, 64))
+    shapes = [tuple(int(v) for v in rng.choice(vals, 3)) for _ in range(16)]
+    probs, host = _problems(fm, shapes, torch.float32, "u11", 4)
+    b = fm.Batch(probs, graph=True)
+    b.run()
+    torch.cuda.synchronize()
+    chosen = [b.choice(i) for i in range(len(probs))]
+    assert {a for a, s in chosen} <= {"classical", "strassen", "grey423_20_144", "smirnov333_23_139"}
+    for (p, (A, B, C0), (alg, st)) in zip(probs, host, chosen):
+        m, n = A.shape[0], B.shape[1]
+        rows, cols = rng.integers(0, m, 256), rng.integers(0, n, 256)
+        q, absab = oracle_mod.quad_entries(A.astype(np.float32).astype(np.float64),
+                                           B.astype(np.float32).astype(np.float64), rows, cols)
+        got = p["C"].cpu().numpy()[rows, cols].astype(np.float64)
+        assert np.all(np.abs(got - q) <= 64 * (1 + st) * 2.0 ** -24 * absab), (alg, st)
+

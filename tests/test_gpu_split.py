@@ -75,7 +75,10 @@ def test_split_accuracy_fp32_level(fm, oracle_mod, dist, alg, steps):
         A, B, _ = make_inputs(m, k, n, 31, "u11", np.float32)
     else:
         A, B = _scaled(m, k, n, 32)
-    D = oracle_mod.quad_matmul(A.astype(np.float64), B.astype(np.float64))
+    # quad-precision values of a sample of 4096 entries
+    rng = np.random.default_rng(7)
+    rows, cols = rng.integers(0, m, 4096), rng.integers(0, n, 4096)
+    D, absab = oracle_mod.quad_entries(A.astype(np.float64), B.astype(np.float64), rows, cols)
     Ad, Bd = _dev(A, fm), _dev(B, fm)
     if alg is None:
         C = fm.gemm(Ad, Bd, mode="split_bf16").cpu().numpy()
@@ -83,10 +86,15 @@ def test_split_accuracy_fp32_level(fm, oracle_mod, dist, alg, steps):
     else:
         C = fm.fastmatmul(alg, Ad, Bd, None, steps, mode="split_bf16").cpu().numpy()
         Cp = fm.fastmatmul(alg, Ad, Bd, None, steps).cpu().numpy()
-    # componentwise: |C - AB| <= c 2^-24 (|A||B|) with c = 8 (1 + levels)
-    absab = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
-    err = np.abs(C.astype(np.float64) - D) / absab
-    errp = np.abs(Cp.astype(np.float64) - D) / absab
-    assert err.max() <= 8 * (1 + steps) * 2.0 ** -24 * 16, (err.max(), errp.max())
-    # and no worse than 4x the plain FP32-MFMA path at the same depth
-    assert err.max() <= 4 * errp.max() + 2.0 ** -24
+    err = np.abs(C[rows, cols].astype(np.float64) - D) / absab
+    errp = np.abs(Cp[rows, cols].astype(np.float64) - D) / absab
+    if dist == "u11" or steps == 0:
+        # componentwise: |C - AB| <= c 2^-24 (|A||B|), c = 128 (1 + levels)
+        assert err.max() <= 128 * (1 + steps) * 2.0 ** -24, (err.max(), errp.max())
+    # fast algorithms are stable only normwise: with rows/columns scaled by
+    # 10^+-3 the small rows lose their digits in S = A11 + A22 whatever the
+    # leaf arithmetic; the split leaves must not add to that loss
+    assert err.max() <= 4 * errp.max() + 2.0 ** -24, (err.max(), errp.max())
+    nw = np.abs(C[rows, cols].astype(np.float64) - D).max() / (np.abs(A).max() * np.abs(B).max() * k)
+    nwp = np.abs(Cp[rows, cols].astype(np.float64) - D).max() / (np.abs(A).max() * np.abs(B).max() * k)
+    assert nw <= 4 * nwp + 2.0 ** -24
