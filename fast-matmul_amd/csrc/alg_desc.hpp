@@ -37,6 +37,14 @@ struct FmOp {
 // ws + out_off + (q * NOUT + o) * out_slab with leading dimension out_ld.
 // W stage: read R products of node q from the same slab layout and write the
 // m x n blocks of ops[q] (C_ij = sum W M (+ beta C_ij)).
+//
+// Two-level S/T stage (one launch for levels l and l+1): ops[q] are the
+// level-l operands, split in bm x bn blocks of rows x cols, each block split
+// again in bm x bn sub-blocks of rows2 x cols2.  The level-(l+1) outputs are
+// never written; the level-(l+2) outputs of child node q * R + r go to
+// ws + out_off + ((q * R + r) * NOUT + o) * out_slab, and the sub-blocks of
+// a materialised child that its own children use unchanged (the "views")
+// go to ws + view_off + ((q * R + r) * NVIEW + slot) * out_slab.
 struct FmStageArgs {
   FmBases bases;
   const FmOp* ops;
@@ -44,6 +52,8 @@ struct FmStageArgs {
   int64_t out_off, out_ld, out_slab;
   double x;
   double beta;
+  int32_t rows2, cols2;
+  int64_t view_off;
 };
 
 typedef hipError_t (*FmStageLaunch)(const FmStageArgs& a, hipStream_t s);
@@ -61,6 +71,14 @@ struct FmAlgDesc {
   const int* b_index;
   double (*b_coef)(int r, double x);
   FmStageLaunch s_d, t_d, w_d, s_f, t_f, w_f;
+  // two-level S/T stages (nullptr where the operand has too many blocks to
+  // hold one element of every sub-block in registers)
+  FmStageLaunch s2_d, t2_d, s2_f, t2_f;
+  // per block of the A (B) side: slot of its view buffer in a two-level
+  // stage, or -1 when no product uses that block unchanged
+  int a_nviews, b_nviews;
+  const int* a_view_slot;
+  const int* b_view_slot;
 };
 
 // generated registry (csrc/gen/registry.cpp): one accessor per algorithm

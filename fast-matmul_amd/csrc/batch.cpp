@@ -191,6 +191,7 @@ fm_status fm_batch_create(fm_dtype dtype, fm_mode mode, const fm_problem* proble
     p.role_c = 6 + 3 * i;
     p.alpha = q.alpha;
     p.beta = q.beta;
+    p.fuse = g_stage_fusion != 0;
     P.ext.push_back(const_cast<void*>(q.A));
     P.ext.push_back(const_cast<void*>(q.B));
     P.ext.push_back(q.C);

@@ -41,8 +41,9 @@ std::string plan_key(int dev, const char* alg, int dt, int mode, int64_t m, int6
                      int64_t lda, int64_t ldb, int64_t ldc, int steps, double x, double alpha,
                      double beta) {
   char buf[512];
-  snprintf(buf, sizeof(buf), "%d|%s|%d|%d|%d|%lld|%lld|%lld|%lld|%lld|%lld|%d|%a|%a|%a", dev,
+  snprintf(buf, sizeof(buf), "%d|%s|%d|%d|%d|%d|%lld|%lld|%lld|%lld|%lld|%lld|%d|%a|%a|%a", dev,
            alg ? alg : "<gemm>", dt, mode, mode == FM_MODE_OZAKI_INT8 ? g_ozaki_slices : 0,
+           g_stage_fusion,
            (long long)m, (long long)k, (long long)n, (long long)lda, (long long)ldb,
            (long long)ldc, steps, x, alpha, beta);
   return buf;
@@ -86,6 +87,7 @@ fm_status build_single(Plan& P, const FmAlgDesc* alg, fm_dtype dt, fm_mode mode,
   p.x = x;
   p.alpha = alpha;
   p.beta = beta;
+  p.fuse = g_stage_fusion != 0;
   std::vector<SubPlan> subs(1);
   fm_status st = build_subplan(dt, p, 0, subs[0]);
   if (st != FM_OK) return st;
@@ -289,6 +291,11 @@ fm_status fm_set_gemm_tile(int cfg) {
   g_tile_override = cfg;
   g_plans.clear();  // plans embed the tile choice
   return FM_OK;
+}
+
+void fm_set_stage_fusion(int on) {
+  std::lock_guard<std::mutex> lock(fm_api_mutex());
+  g_stage_fusion = on != 0;
 }
 
 void fm_profile_enable(int on) {

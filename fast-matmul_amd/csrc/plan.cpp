@@ -35,6 +35,9 @@ const int64_t kSlabAlign = 64;  // elements; keeps every buffer 256 B aligned
 struct Node {
   FmOp a, b, c;
   double mu;
+  // operand produced by a two-level stage and never stored: the workspace
+  // offset of its view sub-blocks (-1 when the operand is stored)
+  int64_t a_views = -1, b_views = -1;
 };
 
 FmOp op_make(int role, int64_t off, int64_t ld) {
@@ -61,6 +64,31 @@ FmGemmDesc gdesc(FmOp a, FmOp b, FmOp c, int64_t M, int64_t N, int64_t K, double
   d.alpha = alpha;
   d.beta = beta;
   return d;
+}
+
+// Two-level S/T stage over the nodes whose operands are `ops` (blocks
+// rows x cols, sub-blocks rows2 x cols2).
+Launch stage2_launch(LaunchKind kind, const FmAlgDesc* alg, std::vector<FmOp> ops, int64_t rows,
+                     int64_t cols, int64_t rows2, int64_t cols2, int64_t out_off, int64_t out_ld,
+                     int64_t out_slab, int64_t view_off, double x) {
+  Launch L;
+  L.kind = kind;
+  L.alg = alg;
+  L.stage = kind == LK_S2 ? FM_STAGE_S : FM_STAGE_T;
+  L.st.rows = (int32_t)rows;
+  L.st.cols = (int32_t)cols;
+  L.st.rows2 = (int32_t)rows2;
+  L.st.cols2 = (int32_t)cols2;
+  L.st.nodes = (int32_t)ops.size();
+  L.st.out_off = out_off;
+  L.st.out_ld = out_ld;
+  L.st.out_slab = out_slab;
+  L.st.view_off = view_off;
+  L.st.x = x;
+  L.st.beta = 0.0;
+  L.vec_ok = 1;
+  L.ops = std::move(ops);
+  return L;
 }
 
 Launch stage_launch(fm_dtype dt, LaunchKind kind, const FmAlgDesc* alg, std::vector<FmOp> ops,
@@ -191,6 +219,7 @@ fm_status ozaki_launch(int S, const std::vector<FmGemmDesc>& prods, size_t* ws_b
 
 int g_tile_override = -1;
 int g_ozaki_slices = 8;
+int g_stage_fusion = 1;
 
 fm_status set_err(fm_status s, const std::string& msg) {
   t_err = msg;
@@ -270,6 +299,26 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
   std::vector<int64_t> m_off(L), m_ld(L), m_slab(L);
   int64_t ws = ws_base;
   const int R = alg->R, NS = alg->ns, NT = alg->nt;
+  // Two-level stages: s2[l] (t2[l]) merges the S (T) stage of level l into
+  // that of level l + 1.  Pairs are taken from the deepest level up (deeper
+  // levels move more data); a pair needs level l + 1 free of peeling, which
+  // would read the level-(l+1) operands that a two-level stage never stores.
+  std::vector<char> s2(L, 0), t2(L, 0);
+  if (p.fuse) {
+    const bool fs = (dt == FM_F64 ? alg->s2_d : alg->s2_f) != nullptr;
+    const bool ft = (dt == FM_F64 ? alg->t2_d : alg->t2_f) != nullptr;
+    for (int l = L - 2; l >= 0;) {
+      const Dims c = dims[l + 1];
+      if (c.m % alg->m == 0 && c.k % alg->k == 0 && c.n % alg->n == 0 && (fs || ft)) {
+        s2[l] = fs;
+        t2[l] = ft;
+        l -= 2;
+      } else {
+        l -= 1;
+      }
+    }
+  }
+  const FmOp virt = op_make(-1, 0, 0);
   for (int l = 0; l < L; ++l) {
     const Dims c = dims[l + 1];
     const int64_t Q = (int64_t)lv[l].size();
@@ -277,38 +326,63 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
     const int64_t t_ld = align_up(c.k, kLdAlign), t_slab = align_up(t_ld * c.n, kSlabAlign);
     m_ld[l] = align_up(c.m, kLdAlign);
     m_slab[l] = align_up(m_ld[l] * c.n, kSlabAlign);
+    const bool s_first = s2[l], s_second = l > 0 && s2[l - 1];
+    const bool t_first = t2[l], t_second = l > 0 && t2[l - 1];
     const int64_t s_off = ws;
-    ws += Q * NS * s_slab;
+    if (!s_first) ws += Q * NS * s_slab;
     const int64_t t_off = ws;
-    ws += Q * NT * t_slab;
+    if (!t_first) ws += Q * NT * t_slab;
+    const int64_t sv_off = ws;
+    if (s_second) ws += Q * alg->a_nviews * s_slab;
+    const int64_t tv_off = ws;
+    if (t_second) ws += Q * alg->b_nviews * t_slab;
     m_off[l] = ws;
     ws += Q * R * m_slab[l];
     lv[l + 1].reserve(Q * R);
     for (int64_t q = 0; q < Q; ++q) {
-      const Node& nd = lv[l][q];
+      Node& nd = lv[l][q];
+      if (s_second && nd.a.role < 0) nd.a_views = sv_off + q * alg->a_nviews * s_slab;
+      if (t_second && nd.b.role < 0) nd.b_views = tv_off + q * alg->b_nviews * t_slab;
       for (int r = 0; r < R; ++r) {
         Node ch;
+        const int ai = alg->a_index[r], bi = alg->b_index[r];
         if (alg->a_kind[r])
-          ch.a = op_make(FM_ROLE_WS, s_off + (q * NS + alg->a_index[r]) * s_slab, s_ld);
+          ch.a = s_first ? virt : op_make(FM_ROLE_WS, s_off + (q * NS + ai) * s_slab, s_ld);
+        else if (nd.a_views >= 0)
+          ch.a = op_make(FM_ROLE_WS, nd.a_views + alg->a_view_slot[ai] * s_slab, s_ld);
         else
-          ch.a = op_sub(nd.a, (alg->a_index[r] / alg->k) * c.m, (alg->a_index[r] % alg->k) * c.k);
+          ch.a = op_sub(nd.a, (ai / alg->k) * c.m, (ai % alg->k) * c.k);
         if (alg->b_kind[r])
-          ch.b = op_make(FM_ROLE_WS, t_off + (q * NT + alg->b_index[r]) * t_slab, t_ld);
+          ch.b = t_first ? virt : op_make(FM_ROLE_WS, t_off + (q * NT + bi) * t_slab, t_ld);
+        else if (nd.b_views >= 0)
+          ch.b = op_make(FM_ROLE_WS, nd.b_views + alg->b_view_slot[bi] * t_slab, t_ld);
         else
-          ch.b = op_sub(nd.b, (alg->b_index[r] / alg->n) * c.k, (alg->b_index[r] % alg->n) * c.n);
+          ch.b = op_sub(nd.b, (bi / alg->n) * c.k, (bi % alg->n) * c.n);
         ch.c = op_make(FM_ROLE_WS, m_off[l] + (q * R + r) * m_slab[l], m_ld[l]);
         double mu = round_scalar(nd.mu * round_scalar(alg->a_coef(r, p.x), dt), dt);
         ch.mu = round_scalar(mu * round_scalar(alg->b_coef(r, p.x), dt), dt);
         lv[l + 1].push_back(ch);
       }
     }
-    if (NS > 0) {
+    if (NS > 0 && s_second) {
+      const Dims pc = dims[l];
+      std::vector<FmOp> ops;
+      for (const Node& nd : lv[l - 1]) ops.push_back(nd.a);
+      out.pre.push_back(stage2_launch(LK_S2, alg, std::move(ops), pc.m, pc.k, c.m, c.k, s_off,
+                                      s_ld, s_slab, sv_off, p.x));
+    } else if (NS > 0 && !s_first) {
       std::vector<FmOp> ops;
       for (const Node& nd : lv[l]) ops.push_back(nd.a);
       out.pre.push_back(stage_launch(dt, LK_S, alg, std::move(ops), c.m, c.k, s_off, s_ld, s_slab,
                                      p.x, 0.0));
     }
-    if (NT > 0) {
+    if (NT > 0 && t_second) {
+      const Dims pc = dims[l];
+      std::vector<FmOp> ops;
+      for (const Node& nd : lv[l - 1]) ops.push_back(nd.b);
+      out.pre.push_back(stage2_launch(LK_T2, alg, std::move(ops), pc.k, pc.n, c.k, c.n, t_off,
+                                      t_ld, t_slab, tv_off, p.x));
+    } else if (NT > 0 && !t_first) {
       std::vector<FmOp> ops;
       for (const Node& nd : lv[l]) ops.push_back(nd.b);
       out.pre.push_back(stage_launch(dt, LK_T, alg, std::move(ops), c.k, c.n, t_off, t_ld, t_slab,
@@ -488,10 +562,14 @@ fm_status Plan::run(const void* A, const void* B, void* C, void* ws, hipStream_t
       a.vec = aligned ? L.vec_ok : 1;
       const FmAlgDesc* g = L.alg;
       FmStageLaunch fn;
-      if (dtype == FM_F64)
-        fn = L.kind == LK_S ? g->s_d : L.kind == LK_T ? g->t_d : g->w_d;
-      else
-        fn = L.kind == LK_S ? g->s_f : L.kind == LK_T ? g->t_f : g->w_f;
+      const bool d = dtype == FM_F64;
+      switch (L.kind) {
+        case LK_S: fn = d ? g->s_d : g->s_f; break;
+        case LK_T: fn = d ? g->t_d : g->t_f; break;
+        case LK_S2: fn = d ? g->s2_d : g->s2_f; break;
+        case LK_T2: fn = d ? g->t2_d : g->t2_f; break;
+        default: fn = d ? g->w_d : g->w_f; break;
+      }
       e = fn(a, s);
     }
     if (g_prof_on && pr.e0 && pr.e1) {

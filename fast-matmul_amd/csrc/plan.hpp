@@ -19,7 +19,7 @@ fm_status hip_err(hipError_t e, const char* what);
 const std::string& last_error();
 void clear_error();
 
-enum LaunchKind { LK_S, LK_T, LK_W, LK_GEMM, LK_OZAKI };
+enum LaunchKind { LK_S, LK_T, LK_W, LK_GEMM, LK_OZAKI, LK_S2, LK_T2 };
 
 // One kernel launch.  Descriptors stay on the host until Plan::upload packs
 // them into the plan's device buffer.
@@ -62,6 +62,7 @@ struct Problem {
   int64_t lda = 0, ldb = 0, ldc = 0;
   int role_a = FM_ROLE_A, role_b = FM_ROLE_B, role_c = FM_ROLE_C;
   double x = 1e-8, alpha = 1.0, beta = 0.0;
+  bool fuse = true;  // two-level S/T stages where the shape allows them
 };
 
 struct Dims {
@@ -93,6 +94,7 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
 int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods);
 extern int g_tile_override;
 extern int g_ozaki_slices;
+extern int g_stage_fusion;
 
 // profiling (fm_profile_enable / fm_profile_read)
 extern bool g_prof_on;

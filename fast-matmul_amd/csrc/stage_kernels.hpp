@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "alg_desc.hpp"
 
 __device__ __forceinline__ double fm_add(double a, double b) { return __dadd_rn(a, b); }
@@ -155,6 +157,100 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage(FmStageArgs a) {
   }
 }
 
+// Compile-time loop: f(std::integral_constant<int, I>) for I = B .. E-1.
+template <int B, int E, class F>
+__device__ __forceinline__ void fm_static_for(F& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>());
+    fm_static_for<B + 1, E>(f);
+  }
+}
+
+// Two recursion levels of S (SIDE 0) or T (SIDE 1) in one pass.  A lane owns
+// one element position of the level-(l+2) blocks and loads it from every
+// sub-block of every block of its node's operand (NB * NB values, each read
+// from HBM once).  For each product r it forms the level-(l+1) operand's
+// sub-block values exactly as the one-level stage would (the same generated
+// expression of output a_index[r], or the block itself when the product
+// uses it unchanged), then applies the level-(l+1) combination to them and
+// writes the outputs, plus the view sub-blocks a materialised operand's
+// children read directly.  Every value is computed by the same operations in
+// the same order as two one-level stages, so results are bit-identical; only
+// the level-(l+1) outputs are never stored.
+template <class Cmb, typename T, int SIDE>
+__global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2(FmStageArgs a) {
+  constexpr int BC = SIDE == 0 ? Cmb::k : Cmb::n;
+  constexpr int NB = (SIDE == 0 ? Cmb::m : Cmb::k) * BC;
+  constexpr int R = Cmb::R;
+  constexpr int NOUT = SIDE == 0 ? Cmb::NS : Cmb::NT;
+  constexpr int NOUT_ARR = NOUT > 0 ? NOUT : 1;
+  constexpr int NV = SIDE == 0 ? Cmb::a_nviews : Cmb::b_nviews;
+  const int q = blockIdx.y;
+  const FmOp op = a.ops[q];
+  const T* src = reinterpret_cast<const T*>(fm_base(a.bases, op.role)) + op.off;
+  T* ws = reinterpret_cast<T*>(a.bases.p[FM_ROLE_WS]);
+  T* dst = ws + a.out_off + (int64_t)q * R * NOUT * a.out_slab;
+  T* vdst = ws + a.view_off + (int64_t)q * R * NV * a.out_slab;
+  const uint32_t rows = (uint32_t)a.rows2;
+  const uint32_t total = rows * (uint32_t)a.cols2;
+  const int64_t ld = op.ld;
+  const double x = a.x;
+  for (uint32_t e = blockIdx.x * FM_STAGE_THREADS + threadIdx.x; e < total;
+       e += gridDim.x * FM_STAGE_THREADS) {
+    const uint32_t j = e / rows;
+    const uint32_t i = e - j * rows;
+    T v[NB][NB];  // v[block][sub-block]
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int sb = 0; sb < NB; ++sb) {
+        const int64_t r0 = (int64_t)(b / BC) * a.rows + (int64_t)(sb / BC) * a.rows2 + i;
+        const int64_t c0 = (int64_t)(b % BC) * a.cols + (int64_t)(sb % BC) * a.cols2 + j;
+        v[b][sb] = src[r0 + c0 * ld];
+      }
+    const int64_t opos = i + (int64_t)j * a.out_ld;
+    auto product = [&](auto rc) {
+      constexpr int r = decltype(rc)::value;
+      constexpr int kind = SIDE == 0 ? Cmb::a_kind_c[r] : Cmb::b_kind_c[r];
+      constexpr int idx = SIDE == 0 ? Cmb::a_index_c[r] : Cmb::b_index_c[r];
+      T par[NB];
+#pragma unroll
+      for (int sb = 0; sb < NB; ++sb) {
+        if constexpr (kind == 0) {
+          par[sb] = v[idx][sb];
+        } else {
+          T in[NB];
+#pragma unroll
+          for (int b = 0; b < NB; ++b) in[b] = v[b][sb];
+          if constexpr (SIDE == 0)
+            par[sb] = Cmb::template s_one<idx, T>(in, x);
+          else
+            par[sb] = Cmb::template t_one<idx, T>(in, x);
+        }
+      }
+      T out[NOUT_ARR];
+      if constexpr (SIDE == 0)
+        Cmb::s_combine(par, out, x);
+      else
+        Cmb::t_combine(par, out, x);
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t) dst[((int64_t)r * NOUT + t) * a.out_slab + opos] = out[t];
+      if constexpr (kind == 1) {
+#pragma unroll
+        for (int sb = 0; sb < NB; ++sb) {
+          const int slot = SIDE == 0 ? Cmb::a_view_slot[sb] : Cmb::b_view_slot[sb];
+          if (slot >= 0) vdst[((int64_t)r * NV + slot) * a.out_slab + opos] = par[sb];
+        }
+      }
+    };
+    fm_static_for<0, R>(product);
+  }
+}
+
+// Operands with at most this many blocks take the two-level stage
+// (NB * NB values per lane in registers).
+constexpr int FM_STAGE2_MAX_BLOCKS = 9;
+
 inline dim3 fm_stage_grid(const FmStageArgs& a, int vec) {
   const uint64_t total = (uint64_t)(a.rows / vec) * (uint64_t)a.cols;
   uint64_t gx = (total + FM_STAGE_THREADS - 1) / FM_STAGE_THREADS;
@@ -201,6 +297,37 @@ hipError_t fm_launch_w_stage(const FmStageArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+inline dim3 fm_stage2_grid(const FmStageArgs& a) {
+  const uint64_t total = (uint64_t)a.rows2 * (uint64_t)a.cols2;
+  uint64_t gx = (total + FM_STAGE_THREADS - 1) / FM_STAGE_THREADS;
+  uint64_t cap = 8192 / (uint64_t)(a.nodes > 0 ? a.nodes : 1);
+  if (cap < 4) cap = 4;
+  if (gx > cap) gx = cap;
+  if (gx < 1) gx = 1;
+  return dim3((unsigned)gx, (unsigned)a.nodes, 1);
+}
+
+template <class Cmb, typename T, int SIDE>
+hipError_t fm_launch_in_stage2(const FmStageArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((fm_in_stage2<Cmb, T, SIDE>), fm_stage2_grid(a), dim3(FM_STAGE_THREADS), 0, s,
+                     a);
+  return hipGetLastError();
+}
+
+template <class Cmb, int SIDE>
+constexpr bool fm_stage2_ok() {
+  return (SIDE == 0 ? Cmb::m * Cmb::k : Cmb::k * Cmb::n) <= FM_STAGE2_MAX_BLOCKS &&
+         (SIDE == 0 ? Cmb::NS : Cmb::NT) > 0;
+}
+
+template <class Cmb, typename T, int SIDE>
+constexpr FmStageLaunch fm_stage2_fn() {
+  if constexpr (fm_stage2_ok<Cmb, SIDE>())
+    return &fm_launch_in_stage2<Cmb, T, SIDE>;
+  else
+    return nullptr;
+}
+
 #define FM_DEFINE_ALGORITHM(NAME, GNS, APA)                                                   \
   static hipError_t fm_##NAME##_s_d(const FmStageArgs& a, hipStream_t s) {                   \
     return fm_launch_in_stage<GNS::Combine, double, 0>(a, s);                                 \
@@ -225,6 +352,10 @@ hipError_t fm_launch_w_stage(const FmStageArgs& a, hipStream_t s) {
         #NAME, GNS::Combine::m, GNS::Combine::k, GNS::Combine::n, GNS::Combine::R,           \
         GNS::Combine::NS, GNS::Combine::NT, APA, GNS::a_kind, GNS::a_index, GNS::a_coef,     \
         GNS::b_kind, GNS::b_index, GNS::b_coef, fm_##NAME##_s_d, fm_##NAME##_t_d,           \
-        fm_##NAME##_w_d, fm_##NAME##_s_f, fm_##NAME##_t_f, fm_##NAME##_w_f};                  \
+        fm_##NAME##_w_d, fm_##NAME##_s_f, fm_##NAME##_t_f, fm_##NAME##_w_f,                   \
+        fm_stage2_fn<GNS::Combine, double, 0>(), fm_stage2_fn<GNS::Combine, double, 1>(),     \
+        fm_stage2_fn<GNS::Combine, float, 0>(), fm_stage2_fn<GNS::Combine, float, 1>(),       \
+        GNS::Combine::a_nviews, GNS::Combine::b_nviews, GNS::Combine::a_view_slot,            \
+        GNS::Combine::b_view_slot};                                                          \
     return &d;                                                                               \
   }

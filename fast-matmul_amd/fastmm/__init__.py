@@ -148,6 +148,11 @@ def set_gemm_tile(cfg: int = -1):
     check(LIB.fm_set_gemm_tile(int(cfg)))
 
 
+def set_stage_fusion(on: bool = True):
+    """Two-level S/T stages where the shape allows them (bit-identical results)."""
+    LIB.fm_set_stage_fusion(1 if on else 0)
+
+
 def set_ozaki_slices(slices: int = 8):
     """Digit count of the ozaki_int8 leaf mode (6, 7 or 8)."""
     check(LIB.fm_set_ozaki_slices(int(slices)))

@@ -31,6 +31,7 @@ EXPORTS = (
     "fm_release_caches", "fm_profile_enable", "fm_profile_read", "fm_set_gemm_tile",
     "fm_set_ozaki_slices", "fm_batch_create", "fm_batch_run", "fm_batch_choice",
     "fm_batch_workspace_bytes", "fm_batch_destroy", "fm_autotune_estimate",
+    "fm_set_stage_fusion",
 )
 
 STAGES = ("S", "T", "W", "LEAF", "PEEL")
@@ -89,6 +90,8 @@ def _load():
     lib.fm_set_gemm_tile.argtypes = [c_int]
     lib.fm_set_ozaki_slices.restype = c_int
     lib.fm_set_ozaki_slices.argtypes = [c_int]
+    lib.fm_set_stage_fusion.restype = None
+    lib.fm_set_stage_fusion.argtypes = [c_int]
     lib.fm_batch_create.restype = c_int
     lib.fm_batch_create.argtypes = [c_int, c_int, ctypes.POINTER(FmProblem), c_int,
                                     ctypes.POINTER(ctypes.c_char_p), c_int, c_int, c_int, vp,

@@ -159,10 +159,19 @@ fm_status fm_profile_read(fm_profile* out);
  * power-of-two scale, and S(S+1)/2 int8 products are summed. */
 fm_status fm_set_ozaki_slices(int slices);
 
-/* Tuning override: force leaf-GEMM tile configuration `cfg` (0 .. 10, see
+/* Tuning override: force leaf-GEMM tile configuration `cfg` (0 .. 13, see
  * FmTileCfg in the sources) for every plan built afterwards; -1 restores the
  * automatic choice.  Cached plans are dropped. */
 fm_status fm_set_gemm_tile(int cfg);
+
+/* Two-level S/T stages (default on): where two consecutive recursion levels
+ * have no peeling between them, one launch forms the deeper level's S (T)
+ * operands straight from the upper level's, without storing the middle
+ * level.  Results are bit-identical either way; off trades speed for the
+ * one-level schedule (plans built afterwards follow the setting).  No
+ * counterpart in the reference, whose adds run one level at a time
+ * (algorithms/strassen.hpp:9-252). */
+void fm_set_stage_fusion(int on);
 
 /* Drop cached plans and the library-owned workspace arena. */
 void fm_release_caches(void);

@@ -64,9 +64,28 @@ def test_argument_errors():
 
 def test_workspace_query_and_depth_limit():
     import fastmm
-    # C2-sq: 14.27 GB of S/T/M buffers for 2 BFS levels of <4,2,4;26> at 8192^3
-    b = fastmm.workspace_bytes("grey424_26_206", 8192, 8192, 8192, 2)
-    assert 14e9 < b < 14.5e9
+    # grey424_26_206 (<4,2,4;26>, 22 S and 20 T outputs, 4 A blocks and 6 B
+    # blocks used unchanged) over 2 levels at 8192^3, in units of 8-byte
+    # elements of the level-2 blocks (512 x 2048 S, 2048 x 512 T, 512 x 512 M)
+    s2, t2, m2 = 512 * 2048, 2048 * 512, 512 * 512
+    m1 = 2048 * 2048
+    one_level = (22 * 2048 * 4096 + 20 * 4096 * 2048 + 26 * m1          # level 1
+                 + 26 * (22 * s2 + 20 * t2) + 676 * m2)                 # level 2
+    two_level = 26 * m1 + 26 * (22 * s2 + 20 * t2) + 26 * (4 * s2 + 6 * t2) + 676 * m2
+    try:
+        fastmm.set_stage_fusion(False)
+        assert fastmm.workspace_bytes("grey424_26_206", 8192, 8192, 8192, 2) == 8 * one_level
+    finally:
+        fastmm.set_stage_fusion(True)
+    assert fastmm.workspace_bytes("grey424_26_206", 8192, 8192, 8192, 2) == 8 * two_level
+    # a peeling level in between keeps the one-level stages: 8200 / 4 = 2050
+    # rows per level-1 block, not a multiple of 4
+    one = fastmm.workspace_bytes("grey424_26_206", 8200, 8192, 8192, 2)
+    fastmm.set_stage_fusion(False)
+    try:
+        assert fastmm.workspace_bytes("grey424_26_206", 8200, 8192, 8192, 2) == one
+    finally:
+        fastmm.set_stage_fusion(True)
     assert fastmm.workspace_bytes("strassen", 64, 64, 64, 0) == 0
     with pytest.raises(ValueError):
         fastmm.workspace_bytes("strassen", 3, 3, 3, 2)
