@@ -263,9 +263,13 @@ def main():
 
     t_tab = fastmm.tables.get_table(alg)
     F_leaf, leaf_dims = leaf_flops(t_tab.R, t_tab.m, t_tab.k, t_tab.n, steps, M, K, N)
+    # the leaf products of one step may run as two launches (full rounds of
+    # large tiles, then the part-empty last round on small tiles): the leaf
+    # kernel's rate is the step's leaf flops over the step's leaf kernel time
     leaf_ms, leaf_n = prof["LEAF"]
-    leaf_avg_ms = leaf_ms / max(1, leaf_n)
-    achieved = F_leaf / (leaf_avg_ms * 1e-3) / 1e12 if leaf_n else None
+    leaf_ms_step = leaf_ms / args.steps
+    achieved = F_leaf / (leaf_ms_step * 1e-3) / 1e12 if leaf_n else None
+    plan = fastmm.plan_info(alg, M, K, N, steps)
     stages = {k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps}
               for k, v in prof.items()}
 
@@ -324,7 +328,9 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "leaf grouped GEMM (fm_gemm_kernel<double>)",
                      "achieved": achieved, "peak": PEAK_FP64_TF, "unit": "TFLOP/s",
                      "frac": (achieved / PEAK_FP64_TF) if achieved else None,
-                     "flops_per_launch": F_leaf, "avg_launch_ms": leaf_avg_ms,
+                     "flops_per_step": F_leaf, "leaf_ms_per_step": leaf_ms_step,
+                     "leaf_launches_per_step": leaf_n / args.steps,
+                     "leaf_launches": plan["leaf"],
                      "traffic": measured_traffic() if args.workload == "c2sq" else None,
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)
                      if args.workload == "c2sq" else None},

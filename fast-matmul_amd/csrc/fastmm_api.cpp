@@ -41,9 +41,9 @@ std::string plan_key(int dev, const char* alg, int dt, int mode, int64_t m, int6
                      int64_t lda, int64_t ldb, int64_t ldc, int steps, double x, double alpha,
                      double beta) {
   char buf[512];
-  snprintf(buf, sizeof(buf), "%d|%s|%d|%d|%d|%d|%lld|%lld|%lld|%lld|%lld|%lld|%d|%a|%a|%a", dev,
-           alg ? alg : "<gemm>", dt, mode, mode == FM_MODE_OZAKI_INT8 ? g_ozaki_slices : 0,
-           g_stage_fusion,
+  snprintf(buf, sizeof(buf), "%d|%s|%d|%d|%d|%d|%d|%d|%zu|%lld|%lld|%lld|%lld|%lld|%lld|%d|%a|%a|%a",
+           dev, alg ? alg : "<gemm>", dt, mode, mode == FM_MODE_OZAKI_INT8 ? g_ozaki_slices : 0,
+           g_stage_fusion, g_split_remainder, g_dfs_levels, g_ws_limit,
            (long long)m, (long long)k, (long long)n, (long long)lda, (long long)ldb,
            (long long)ldc, steps, x, alpha, beta);
   return buf;
@@ -88,10 +88,33 @@ fm_status build_single(Plan& P, const FmAlgDesc* alg, fm_dtype dt, fm_mode mode,
   p.alpha = alpha;
   p.beta = beta;
   p.fuse = g_stage_fusion != 0;
-  std::vector<SubPlan> subs(1);
-  fm_status st = build_subplan(dt, p, 0, subs[0]);
-  if (st != FM_OK) return st;
-  return assemble(P, subs, subs[0].ws_elems);
+  p.mode = mode;
+  p.slices = P.slices;
+  // depth-first top levels: as many as asked, then more from the top while
+  // the plan's workspace exceeds the limit (the reference's DFS / HYBRID
+  // schedules, util/par_util.hpp:17-95, chosen per plan instead of at
+  // compile time)
+  for (p.dfs = alg ? std::min(g_dfs_levels, steps) : 0;; ++p.dfs) {
+    Plan trial;
+    trial.dtype = P.dtype;
+    trial.mode = P.mode;
+    trial.slices = P.slices;
+    std::vector<SubPlan> subs(1);
+    fm_status st = build_subplan(dt, p, 0, subs[0]);
+    if (st != FM_OK) return st;
+    st = assemble(trial, subs, subs[0].ws_elems);
+    if (st != FM_OK) return st;
+    const bool last = !alg || p.dfs >= steps;
+    if (g_ws_limit == 0 || trial.ws_bytes <= g_ws_limit || last) {
+      if (g_ws_limit > 0 && trial.ws_bytes > g_ws_limit)
+        return set_err(FM_ERR_WORKSPACE,
+                       "workspace limit below the depth-first plan's " +
+                           std::to_string(trial.ws_bytes) + " bytes");
+      P.launches = std::move(trial.launches);
+      P.ws_bytes = trial.ws_bytes;
+      return FM_OK;
+    }
+  }
 }
 
 fm_status get_plan(const char* alg_name, const FmAlgDesc* alg, fm_dtype dt, fm_mode mode, int m,
@@ -226,6 +249,39 @@ fm_status fm_workspace_query(const char* alg_name, fm_dtype dtype, fm_mode mode,
   return FM_OK;
 }
 
+fm_status fm_plan_query(const char* alg_name, fm_dtype dtype, fm_mode mode, int m, int k, int n,
+                        int num_steps, fm_plan_info* info) {
+  clear_error();
+  if (!info) return set_err(FM_ERR_INVALID_ARG, "null output");
+  const FmAlgDesc* alg = fm_find_alg(alg_name);
+  if (!alg) return set_err(FM_ERR_UNKNOWN_ALG, "unknown algorithm");
+  if (m <= 0 || k <= 0 || n <= 0 || num_steps < 0) return set_err(FM_ERR_INVALID_ARG, "bad shape");
+  fm_status st = check_mode(dtype, mode);
+  if (st != FM_OK) return st;
+  std::lock_guard<std::mutex> lock(fm_api_mutex());
+  Plan P;
+  st = build_single(P, alg, dtype, mode, m, k, n, m, k, m, num_steps, 1e-8, 1.0, 0.0);
+  if (st != FM_OK) return st;
+  memset(info, 0, sizeof(*info));
+  info->workspace_bytes = P.ws_bytes;
+  int nleaf = 0;
+  for (const Launch& L : P.launches) {
+    info->launches[L.stage] += 1;
+    if (L.kind == LK_S2 || L.kind == LK_T2) info->two_level_stages += 1;
+    if (L.stage == FM_STAGE_LEAF) {
+      const int tiles = L.kind == LK_OZAKI ? L.oz.total_tiles : L.gm.total_tiles;
+      const int prods = L.kind == LK_OZAKI ? L.oz.ndesc : L.gm.ndesc;
+      if (nleaf < 2) {
+        info->leaf_cfg[nleaf] = L.kind == LK_OZAKI ? -1 : L.cfg;
+        info->leaf_products[nleaf] = prods;
+        info->leaf_tiles[nleaf] = tiles;
+      }
+      ++nleaf;
+    }
+  }
+  return FM_OK;
+}
+
 fm_status fm_gemm(fm_dtype dtype, fm_mode mode, int m, int k, int n, const void* A, int64_t lda,
                   const void* B, int64_t ldb, void* C, int64_t ldc, double alpha, double beta,
                   void* stream) {
@@ -290,6 +346,15 @@ fm_status fm_set_gemm_tile(int cfg) {
   if (cfg < -1 || cfg >= FM_TILE_NUM) return set_err(FM_ERR_INVALID_ARG, "unknown tile config");
   g_tile_override = cfg;
   g_plans.clear();  // plans embed the tile choice
+  return FM_OK;
+}
+
+fm_status fm_set_schedule(int dfs_levels, size_t workspace_limit, int split_remainder) {
+  std::lock_guard<std::mutex> lock(fm_api_mutex());
+  if (dfs_levels < 0) return set_err(FM_ERR_INVALID_ARG, "dfs_levels must be >= 0");
+  g_dfs_levels = dfs_levels;
+  g_ws_limit = workspace_limit;
+  g_split_remainder = split_remainder != 0;
   return FM_OK;
 }
 

@@ -52,8 +52,13 @@ __device__ __forceinline__ int fm_xcd_remap(int wg, int nwg) {
   return wg;
 }
 
-template <typename T, int BM, int BN, int BK, int WM, int WN>
-__global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) {
+// MINW: minimum waves per SIMD the register allocation must allow (2 with
+// four-wave work groups lets two work groups share a CU, so one group's
+// barrier wait is covered by the other's MFMAs).  UNR: unroll of the
+// 4-deep K steps inside one LDS slice (less unrolling, fewer live operand
+// registers).
+template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW = 1, int UNR = BK / 4>
+__global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs args) {
   using Op = FmMfma<T>;
   using acc_t = typename Op::acc_t;
   constexpr int NT = 64 * WM * WN;
@@ -206,7 +211,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
       else
         gload((kt + 1) * BK);
     }
-#pragma unroll
+#pragma unroll UNR
     for (int k4 = 0; k4 < BK / 4; ++k4) {
       const int kidx = k4 * 4 + (lane >> 4);
       T af[TM], bf[TN];
@@ -250,11 +255,11 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_kernel(FmGemmArgs args) 
   }  // tile loop
 }
 
-template <typename T, int BM, int BN, int BK, int WM, int WN>
+template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW = 1, int UNR = BK / 4>
 static void fm_gemm_go(const FmGemmArgs& a, hipStream_t s) {
   const int grid = a.grid > 0 && a.grid < a.total_tiles ? a.grid : a.total_tiles;
-  hipLaunchKernelGGL((fm_gemm_kernel<T, BM, BN, BK, WM, WN>), dim3(grid), dim3(64 * WM * WN), 0, s,
-                     a);
+  hipLaunchKernelGGL((fm_gemm_kernel<T, BM, BN, BK, WM, WN, MINW, UNR>), dim3(grid),
+                     dim3(64 * WM * WN), 0, s, a);
 }
 
 template <typename T>
@@ -275,15 +280,21 @@ static hipError_t fm_gemm_launch(int cfg, const FmGemmArgs& a, hipStream_t s) {
     case FM_TILE_128x128_W16: fm_gemm_go<T, 128, 128, 16, 4, 4>(a, s); break;
     case FM_TILE_128x64_W8: fm_gemm_go<T, 128, 64, 16, 2, 4>(a, s); break;
     case FM_TILE_64x128_W8: fm_gemm_go<T, 64, 128, 16, 4, 2>(a, s); break;
+    case FM_TILE_128x128_O2: fm_gemm_go<T, 128, 128, 16, 2, 2, 2>(a, s); break;
+    case FM_TILE_256x128_W8_U1: fm_gemm_go<T, 256, 128, 16, 4, 2, 1, 1>(a, s); break;
+    case FM_TILE_128x128_O2_U1: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 1>(a, s); break;
+    case FM_TILE_128x256_W8_U1: fm_gemm_go<T, 128, 256, 16, 2, 4, 1, 1>(a, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 hipError_t fm_gemm_launch_f64(int cfg, const FmGemmArgs& a, hipStream_t s) {
+  if (cfg >= FM_TILE_DV_128x128) return fm_gemm_dv_launch_f64(cfg, a, s);
   return fm_gemm_launch<double>(cfg, a, s);
 }
 
 hipError_t fm_gemm_launch_f32(int cfg, const FmGemmArgs& a, hipStream_t s) {
+  if (cfg >= FM_TILE_DV_128x128) return fm_gemm_dv_launch_f32(cfg, a, s);
   return fm_gemm_launch<float>(cfg, a, s);
 }

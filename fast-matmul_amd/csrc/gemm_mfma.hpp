@@ -45,7 +45,16 @@ enum FmTileCfg : int {
   FM_TILE_128x128_W16 = 11,    // BK 16, 4x4 waves
   FM_TILE_128x64_W8 = 12,      // BK 16, 2x4 waves
   FM_TILE_64x128_W8 = 13,      // BK 16, 4x2 waves
-  FM_TILE_NUM = 14
+  FM_TILE_128x128_O2 = 14,     // BK 16, 2x2 waves, registers for 2 waves per SIMD
+  FM_TILE_256x128_W8_U1 = 15,  // BK 16, 4x2 waves, K steps not unrolled
+  FM_TILE_128x128_O2_U1 = 16,  // as 14, K steps not unrolled
+  FM_TILE_128x256_W8_U1 = 17,  // BK 16, 2x4 waves, K steps not unrolled
+  // B straight to registers, only A through LDS (csrc/gemm_dv.hip)
+  FM_TILE_DV_128x128 = 18,     // 2x2 waves (64 x 64 each), two work groups per CU
+  FM_TILE_DV_128x128_W8 = 19,  // 2x4 waves (64 x 32 each)
+  FM_TILE_DV_256x128_W16 = 20, // 4x4 waves (64 x 32 each)
+  FM_TILE_DV_128x64 = 21,      // 2x2 waves (64 x 32 each), two work groups per CU
+  FM_TILE_NUM = 22
 };
 
 // output tile of each configuration
@@ -53,11 +62,15 @@ inline void fm_tile_dims(int cfg, int* bm, int* bn) {
   static const int d[FM_TILE_NUM][2] = {{128, 128}, {64, 64}, {128, 64}, {64, 128},
                                         {128, 128}, {128, 128}, {128, 128}, {256, 128},
                                         {128, 256}, {256, 256}, {64, 64},  {128, 128},
-                                        {128, 64},  {64, 128}};
+                                        {128, 64},  {64, 128},  {128, 128}, {256, 128},
+                                        {128, 128}, {128, 256}, {128, 128}, {128, 128},
+                                        {256, 128}, {128, 64}};
   *bm = d[cfg][0];
   *bn = d[cfg][1];
 }
 
+hipError_t fm_gemm_dv_launch_f64(int cfg, const FmGemmArgs& a, hipStream_t s);
+hipError_t fm_gemm_dv_launch_f32(int cfg, const FmGemmArgs& a, hipStream_t s);
 hipError_t fm_gemm_launch_f64(int cfg, const FmGemmArgs& a, hipStream_t s);
 hipError_t fm_gemm_launch_f32(int cfg, const FmGemmArgs& a, hipStream_t s);
 // FP32 data, split-bf16 slice products (csrc/gemm_split.hip)

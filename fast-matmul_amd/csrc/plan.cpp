@@ -220,6 +220,8 @@ fm_status ozaki_launch(int S, const std::vector<FmGemmDesc>& prods, size_t* ws_b
 int g_tile_override = -1;
 int g_ozaki_slices = 8;
 int g_stage_fusion = 1;
+int g_dfs_levels = 0;
+size_t g_ws_limit = 0;
 
 fm_status set_err(fm_status s, const std::string& msg) {
   t_err = msg;
@@ -239,33 +241,72 @@ int elem_size(fm_dtype dt) { return dt == FM_F64 ? 8 : 4; }
 // where a configuration's throughput per tile (eff, measured on MI355X at
 // 8192^3) and its resident work groups per CU set how many rounds of tiles
 // the launch takes.
+namespace {
+struct TileCand {
+  int cfg;
+  double eff;  // throughput per tile area, relative
+  int per_cu;  // resident work groups per CU
+};
+const TileCand kF64Tiles[] = {{FM_TILE_256x128_W8, 1.00, 1}, {FM_TILE_128x256_W8, 0.99, 1},
+                              {FM_TILE_128x128_W16, 0.93, 2}, {FM_TILE_128x128_W8, 0.90, 2},
+                              {FM_TILE_128x64_W8, 0.91, 3},  {FM_TILE_64x64, 0.80, 4}};
+const TileCand kF32Tiles[] = {{FM_TILE_256x256_W16, 1.00, 1}, {FM_TILE_256x128_W8, 0.90, 2},
+                              {FM_TILE_128x256_W8, 0.90, 2},  {FM_TILE_128x128_W8, 0.82, 4},
+                              {FM_TILE_64x128, 0.77, 4},      {FM_TILE_128x64, 0.74, 4},
+                              {FM_TILE_64x64, 0.69, 8}};
+const TileCand kSplitTiles[] = {{FM_TILE_128x128, 1.00, 2}, {FM_TILE_64x64, 0.80, 4}};
+
+const TileCand* tile_cands(fm_dtype dt, fm_mode mode, int* n) {
+  if (mode == FM_MODE_SPLIT_BF16) {
+    *n = 2;
+    return kSplitTiles;
+  }
+  if (dt == FM_F64) {
+    *n = 6;
+    return kF64Tiles;
+  }
+  *n = 7;
+  return kF32Tiles;
+}
+
+int64_t count_tiles(int cfg, const std::vector<FmGemmDesc>& prods, size_t from = 0) {
+  int bm, bn;
+  fm_tile_dims(cfg, &bm, &bn);
+  int64_t tiles = 0;
+  for (size_t i = from; i < prods.size(); ++i)
+    tiles += ((prods[i].M + bm - 1) / bm) * ((prods[i].N + bn - 1) / bn);
+  return tiles;
+}
+
+// Estimated time of one launch of `tiles` tiles: rounds of per_cu work
+// groups on each of the 256 CUs.
+double tile_time(const TileCand& c, int64_t tiles) {
+  int bm, bn;
+  fm_tile_dims(c.cfg, &bm, &bn);
+  const int64_t slots = 256LL * c.per_cu;
+  const double rounds = (double)((tiles + slots - 1) / slots);
+  return rounds * bm * bn * c.per_cu / c.eff;
+}
+
+const TileCand* find_cand(fm_dtype dt, fm_mode mode, int cfg) {
+  int n;
+  const TileCand* c = tile_cands(dt, mode, &n);
+  for (int i = 0; i < n; ++i)
+    if (c[i].cfg == cfg) return &c[i];
+  return nullptr;
+}
+}  // namespace
+
+int g_split_remainder = 1;
+
 int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods) {
   if (g_tile_override >= 0) return g_tile_override;
-  struct C {
-    int cfg;
-    double eff;
-    int per_cu;
-  };
-  static const C f64c[] = {{FM_TILE_256x128_W8, 1.00, 1}, {FM_TILE_128x256_W8, 0.99, 1},
-                           {FM_TILE_128x128_W16, 0.93, 2}, {FM_TILE_128x128_W8, 0.90, 2},
-                           {FM_TILE_128x64_W8, 0.91, 3},  {FM_TILE_64x64, 0.80, 4}};
-  static const C f32c[] = {{FM_TILE_256x256_W16, 1.00, 1}, {FM_TILE_256x128_W8, 0.90, 2},
-                           {FM_TILE_128x256_W8, 0.90, 2},  {FM_TILE_128x128_W8, 0.82, 4},
-                           {FM_TILE_64x128, 0.77, 4},      {FM_TILE_128x64, 0.74, 4},
-                           {FM_TILE_64x64, 0.69, 8}};
-  static const C splitc[] = {{FM_TILE_128x128, 1.00, 2}, {FM_TILE_64x64, 0.80, 4}};
-  const C* cands = mode == FM_MODE_SPLIT_BF16 ? splitc : dt == FM_F64 ? f64c : f32c;
-  const int nc = mode == FM_MODE_SPLIT_BF16 ? 2 : dt == FM_F64 ? 6 : 7;
+  int nc;
+  const TileCand* cands = tile_cands(dt, mode, &nc);
   int best = FM_TILE_64x64;
   double best_t = 1e300;
   for (int i = 0; i < nc; ++i) {
-    int bm, bn;
-    fm_tile_dims(cands[i].cfg, &bm, &bn);
-    int64_t tiles = 0;
-    for (const FmGemmDesc& d : prods) tiles += ((d.M + bm - 1) / bm) * ((d.N + bn - 1) / bn);
-    const int64_t slots = 256LL * cands[i].per_cu;
-    const double rounds = (double)((tiles + slots - 1) / slots);
-    const double t = rounds * bm * bn * cands[i].per_cu / cands[i].eff;
+    const double t = tile_time(cands[i], count_tiles(cands[i].cfg, prods));
     if (t < best_t * 0.999) {
       best_t = t;
       best = cands[i].cfg;
@@ -274,10 +315,126 @@ int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods) {
   return best;
 }
 
+// The GPU form of the reference's HYBRID remainder rule (util/par_util.hpp:
+// 65-95: leaf tasks that do not fill the last round of threads run one at a
+// time on all threads): when the products' tiles leave the last round of
+// work groups part-empty, the products of that round move to a second
+// launch with the tile that finishes them soonest.  Products stay whole, so
+// every output element is computed by the same instructions either way.
+void split_remainder(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods,
+                     std::vector<FmGemmDesc>& rest, int* cfg_rest) {
+  if (g_tile_override >= 0 || P.mode == FM_MODE_OZAKI_INT8) return;
+  const TileCand* c = find_cand(P.dtype, P.mode, cfg);
+  if (!c) return;
+  const int64_t slots = 256LL * c->per_cu;
+  const int64_t total = count_tiles(cfg, prods);
+  const int64_t full = total / slots * slots;
+  if (full == 0 || full == total) return;
+  // the longest prefix of products whose tiles fill whole rounds
+  int bm, bn;
+  fm_tile_dims(cfg, &bm, &bn);
+  int64_t acc = 0;
+  size_t cut = 0;
+  while (cut < prods.size()) {
+    const int64_t t = ((prods[cut].M + bm - 1) / bm) * ((prods[cut].N + bn - 1) / bn);
+    if (acc + t > full) break;
+    acc += t;
+    ++cut;
+  }
+  if (acc != full || cut == prods.size()) return;
+  std::vector<FmGemmDesc> tail(prods.begin() + (std::ptrdiff_t)cut, prods.end());
+  const int cr = pick_tile(P.dtype, P.mode, tail);
+  const TileCand* ct = find_cand(P.dtype, P.mode, cr);
+  if (!ct) return;
+  const double t_one = tile_time(*c, total);
+  const double t_two = tile_time(*c, full) + tile_time(*ct, count_tiles(cr, tail));
+  if (t_two >= t_one * 0.99) return;
+  prods.resize(cut);
+  rest = std::move(tail);
+  *cfg_rest = cr;
+}
+
+// Depth-first top level (the reference's DFS order, algorithms/strassen.hpp:
+// 404-463): the level-0 S/T stages, then each child's whole sub-plan in turn
+// (its own stages, leaf launch and W stages), every child reusing one
+// workspace region, then the level-0 W stage and peeling.  Arithmetic is
+// that of the breadth-first plan; only the launch order and the workspace
+// differ.
+fm_status build_dfs(fm_dtype dt, const Problem& p, const std::vector<Dims>& dims, int64_t ws_base,
+                    SubPlan& out) {
+  const FmAlgDesc* alg = p.alg;
+  const int R = alg->R, NS = alg->ns, NT = alg->nt;
+  const FmOp A0 = op_make(p.role_a, p.off_a, p.lda), B0 = op_make(p.role_b, p.off_b, p.ldb),
+             C0 = op_make(p.role_c, p.off_c, p.ldc);
+  const Dims c = dims[1];
+  const int64_t s_ld = align_up(c.m, kLdAlign), s_slab = align_up(s_ld * c.k, kSlabAlign);
+  const int64_t t_ld = align_up(c.k, kLdAlign), t_slab = align_up(t_ld * c.n, kSlabAlign);
+  const int64_t m_ld = align_up(c.m, kLdAlign), m_slab = align_up(m_ld * c.n, kSlabAlign);
+  int64_t ws = ws_base;
+  const int64_t s_off = ws;
+  ws += NS * s_slab;
+  const int64_t t_off = ws;
+  ws += NT * t_slab;
+  const int64_t m_off = ws;
+  ws += R * m_slab;
+  if (NS > 0)
+    out.pre.push_back(stage_launch(dt, LK_S, alg, {A0}, c.m, c.k, s_off, s_ld, s_slab, p.x, 0.0));
+  if (NT > 0)
+    out.pre.push_back(stage_launch(dt, LK_T, alg, {B0}, c.k, c.n, t_off, t_ld, t_slab, p.x, 0.0));
+  const int64_t child_base = ws;
+  const size_t es = (size_t)elem_size(dt);
+  size_t end_bytes = (size_t)ws * es;
+  const double mu0 = round_scalar(p.alpha, dt);
+  for (int r = 0; r < R; ++r) {
+    const int ai = alg->a_index[r], bi = alg->b_index[r];
+    const FmOp a = alg->a_kind[r] ? op_make(FM_ROLE_WS, s_off + ai * s_slab, s_ld)
+                                  : op_sub(A0, (ai / alg->k) * c.m, (ai % alg->k) * c.k);
+    const FmOp b = alg->b_kind[r] ? op_make(FM_ROLE_WS, t_off + bi * t_slab, t_ld)
+                                  : op_sub(B0, (bi / alg->n) * c.k, (bi % alg->n) * c.n);
+    Problem q = p;
+    q.steps = p.steps - 1;
+    q.dfs = p.dfs - 1;
+    q.m = c.m;
+    q.k = c.k;
+    q.n = c.n;
+    q.role_a = a.role;
+    q.off_a = a.off;
+    q.lda = a.ld;
+    q.role_b = b.role;
+    q.off_b = b.off;
+    q.ldb = b.ld;
+    q.role_c = FM_ROLE_WS;
+    q.off_c = m_off + r * m_slab;
+    q.ldc = m_ld;
+    const double mu = round_scalar(mu0 * round_scalar(alg->a_coef(r, p.x), dt), dt);
+    q.alpha = round_scalar(mu * round_scalar(alg->b_coef(r, p.x), dt), dt);
+    q.beta = 0.0;
+    std::vector<SubPlan> child(1);
+    fm_status st = build_subplan(dt, q, child_base, child[0]);
+    if (st != FM_OK) return st;
+    Plan seq;
+    seq.dtype = dt;
+    seq.mode = p.mode;
+    seq.slices = p.slices;
+    st = assemble(seq, child, child_base + child[0].ws_elems);
+    if (st != FM_OK) return st;
+    for (Launch& L : seq.launches) out.pre.push_back(std::move(L));
+    end_bytes = std::max(end_bytes, seq.ws_bytes);
+  }
+  const double bl = round_scalar(p.beta, dt);
+  out.w.push_back(stage_launch(dt, LK_W, alg, {C0}, c.m, c.n, m_off, m_ld, m_slab, p.x, bl));
+  out.peels.resize(1);
+  peel_descs(Node{A0, B0, C0, mu0}, dims[0], alg->m, alg->k, alg->n, bl, out.peels[0]);
+  out.depth = 1;
+  out.leaf = dims.back();
+  out.ws_elems = (int64_t)((end_bytes + es - 1) / es) - ws_base;
+  return FM_OK;
+}
+
 fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan& out) {
   out = SubPlan();
-  const FmOp A0 = op_make(p.role_a, 0, p.lda), B0 = op_make(p.role_b, 0, p.ldb),
-             C0 = op_make(p.role_c, 0, p.ldc);
+  const FmOp A0 = op_make(p.role_a, p.off_a, p.lda), B0 = op_make(p.role_b, p.off_b, p.ldb),
+             C0 = op_make(p.role_c, p.off_c, p.ldc);
   if (!p.alg) {  // classical product, beta honoured (MatMul, linalg/linalg.hpp:261-269)
     out.leaves.push_back(gdesc(A0, B0, C0, p.m, p.n, p.k, round_scalar(p.alpha, dt),
                                round_scalar(p.beta, dt)));
@@ -294,6 +451,7 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
       return set_err(FM_ERR_SHAPE, "num_steps too deep for this shape: a block dimension "
                                    "reaches 0 at level " + std::to_string(l + 1));
   }
+  if (p.dfs > 0 && L > 0) return build_dfs(dt, p, dims, ws_base, out);
   std::vector<std::vector<Node>> lv(L + 1);
   lv[0].push_back(Node{A0, B0, C0, round_scalar(p.alpha, dt)});
   std::vector<int64_t> m_off(L), m_ld(L), m_slab(L);
@@ -422,8 +580,12 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
       P.launches.push_back(std::move(L));
     } else {
       const int cfg = pick_tile(P.dtype, P.mode, leaves);
+      std::vector<FmGemmDesc> rest;
+      int cfg_rest = cfg;
+      if (persistent_grid == 0 && g_split_remainder) split_remainder(P, cfg, leaves, rest, &cfg_rest);
       P.launches.push_back(gemm_launch(std::move(leaves), cfg, FM_STAGE_LEAF, P.dtype));
       P.launches.back().gm.grid = persistent_grid;
+      if (!rest.empty()) P.launches.push_back(gemm_launch(std::move(rest), cfg_rest, FM_STAGE_LEAF, P.dtype));
     }
   }
   int maxd = 0;

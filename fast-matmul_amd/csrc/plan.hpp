@@ -62,7 +62,11 @@ struct Problem {
   int64_t lda = 0, ldb = 0, ldc = 0;
   int role_a = FM_ROLE_A, role_b = FM_ROLE_B, role_c = FM_ROLE_C;
   double x = 1e-8, alpha = 1.0, beta = 0.0;
+  int64_t off_a = 0, off_b = 0, off_c = 0;  // element offsets within the roles
   bool fuse = true;  // two-level S/T stages where the shape allows them
+  int dfs = 0;       // top levels run depth-first (one child sub-plan at a time)
+  fm_mode mode = FM_MODE_PLAIN;  // leaf mode and digit count, for depth-first children
+  int slices = 0;
 };
 
 struct Dims {
@@ -83,6 +87,8 @@ struct SubPlan {
 
 // Build the sub-plan of `p`, placing its workspace at element offset ws_base.
 fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan& out);
+fm_status build_dfs(fm_dtype dt, const Problem& p, const std::vector<Dims>& dims, int64_t ws_base,
+                    SubPlan& out);
 
 // Merge sub-plans into one launch sequence:
 //   all S/T stages; one grouped leaf launch; per level (deepest first) the W
@@ -92,6 +98,12 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
 fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int persistent_grid = 0);
 
 int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods);
+// move the products of a part-empty last round of tiles to `rest`
+void split_remainder(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods,
+                     std::vector<FmGemmDesc>& rest, int* cfg_rest);
+extern int g_split_remainder;
+extern int g_dfs_levels;
+extern size_t g_ws_limit;
 extern int g_tile_override;
 extern int g_ozaki_slices;
 extern int g_stage_fusion;

@@ -23,7 +23,7 @@ import torch
 
 from . import _lib
 from . import tables
-from ._lib import LIB, check
+from ._lib import LIB, FastMMError, check  # noqa: F401
 
 __all__ = ["Matrix", "fastmatmul", "gemm", "MatMul", "RunAlgorithm", "alg", "algorithm_names",
            "workspace_bytes", "release_caches"]
@@ -146,6 +146,29 @@ def release_caches():
 def set_gemm_tile(cfg: int = -1):
     """Force a leaf-GEMM tile configuration (-1: automatic choice)."""
     check(LIB.fm_set_gemm_tile(int(cfg)))
+
+
+def plan_info(alg_name: str, m: int, k: int, n: int, num_steps: int = 1,
+              dtype=torch.float64, mode: str = "plain") -> dict:
+    """Structure of the plan fastmatmul would run for this shape (built on
+    the host; nothing is launched)."""
+    info = _lib.FmPlanInfo()
+    check(LIB.fm_plan_query(alg_name.encode(), _DT[dtype], _lib.MODES[mode], int(m), int(k),
+                            int(n), int(num_steps), ctypes.byref(info)))
+    return {"launches": {nm: info.launches[i] for i, nm in enumerate(_lib.STAGES)},
+            "two_level_stages": info.two_level_stages,
+            "leaf": [{"cfg": info.leaf_cfg[i], "products": info.leaf_products[i],
+                      "tiles": info.leaf_tiles[i]} for i in range(min(2, info.launches[3]))],
+            "workspace_bytes": info.workspace_bytes}
+
+
+def set_schedule(dfs_levels: int = 0, workspace_limit: int = 0, split_remainder: bool = True):
+    """Recursion schedule for plans built afterwards: the top ``dfs_levels``
+    levels depth-first (one child sub-plan at a time, shared workspace), more
+    depth-first levels while the workspace exceeds ``workspace_limit`` bytes
+    (0: no limit), and the leaf launch's part-empty last round of tiles in a
+    second launch with smaller tiles (``split_remainder``)."""
+    check(LIB.fm_set_schedule(int(dfs_levels), int(workspace_limit), 1 if split_remainder else 0))
 
 
 def set_stage_fusion(on: bool = True):

@@ -31,10 +31,16 @@ EXPORTS = (
     "fm_release_caches", "fm_profile_enable", "fm_profile_read", "fm_set_gemm_tile",
     "fm_set_ozaki_slices", "fm_batch_create", "fm_batch_run", "fm_batch_choice",
     "fm_batch_workspace_bytes", "fm_batch_destroy", "fm_autotune_estimate",
-    "fm_set_stage_fusion",
+    "fm_set_stage_fusion", "fm_set_schedule", "fm_plan_query",
 )
 
 STAGES = ("S", "T", "W", "LEAF", "PEEL")
+
+
+class FmPlanInfo(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_int * 5), ("two_level_stages", ctypes.c_int),
+                ("leaf_cfg", ctypes.c_int * 2), ("leaf_products", ctypes.c_int * 2),
+                ("leaf_tiles", ctypes.c_int * 2), ("workspace_bytes", ctypes.c_size_t)]
 
 
 class FmProfile(ctypes.Structure):
@@ -92,6 +98,11 @@ def _load():
     lib.fm_set_ozaki_slices.argtypes = [c_int]
     lib.fm_set_stage_fusion.restype = None
     lib.fm_set_stage_fusion.argtypes = [c_int]
+    lib.fm_set_schedule.restype = c_int
+    lib.fm_plan_query.restype = c_int
+    lib.fm_plan_query.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int, c_int,
+                                  ctypes.POINTER(FmPlanInfo)]
+    lib.fm_set_schedule.argtypes = [c_int, ctypes.c_size_t, c_int]
     lib.fm_batch_create.restype = c_int
     lib.fm_batch_create.argtypes = [c_int, c_int, ctypes.POINTER(FmProblem), c_int,
                                     ctypes.POINTER(ctypes.c_char_p), c_int, c_int, c_int, vp,

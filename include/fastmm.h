@@ -164,6 +164,39 @@ fm_status fm_set_ozaki_slices(int slices);
  * automatic choice.  Cached plans are dropped. */
 fm_status fm_set_gemm_tile(int cfg);
 
+/* Shape of the plan fm_fastmatmul would run (built on the host, nothing
+ * launched; contiguous operands): launches per fm_stage_kind, how many of
+ * the S/T launches are two-level stages, and the first two leaf launches'
+ * tile configuration (-1: int8-digit leaves), product and tile counts. */
+typedef struct fm_plan_info {
+  int launches[5];
+  int two_level_stages;
+  int leaf_cfg[2];
+  int leaf_products[2];
+  int leaf_tiles[2];
+  size_t workspace_bytes;
+} fm_plan_info;
+fm_status fm_plan_query(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k, int n,
+                        int num_steps, fm_plan_info* info);
+
+/* Recursion schedule, the run-time form of the reference's compile-time
+ * _PARALLEL_ choice of DFS, BFS or HYBRID (util/par_util.hpp:17-95,
+ * algorithms/strassen.hpp:404-463); plans built afterwards follow it.
+ *  - dfs_levels = 0 (default): breadth-first, every level one launch per
+ *    stage over all of its nodes and one grouped launch of all leaves.
+ *  - dfs_levels = d > 0: the top d levels run depth-first, one child at a
+ *    time (its whole breadth-first sub-plan), the children reusing one
+ *    workspace region: less workspace, smaller leaf launches.
+ *  - workspace_limit > 0: depth-first levels are added from the top until
+ *    the plan's workspace fits the limit (FM_ERR_WORKSPACE if even a fully
+ *    depth-first plan does not).
+ *  - split_remainder != 0 (default): when the leaf tiles leave the last
+ *    round of work groups part-empty, those products run in a second
+ *    launch with smaller tiles (the HYBRID rule of running the leftover
+ *    leaves with all threads).
+ * Results are bit-identical under every schedule. */
+fm_status fm_set_schedule(int dfs_levels, size_t workspace_limit, int split_remainder);
+
 /* Two-level S/T stages (default on): where two consecutive recursion levels
  * have no peeling between them, one launch forms the deeper level's S (T)
  * operands straight from the upper level's, without storing the middle

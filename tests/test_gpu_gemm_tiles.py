@@ -1,0 +1,50 @@
+"""Every leaf-GEMM tile configuration, forced through fm_set_gemm_tile, on
+integer data (bit-exact: every partial sum is an exact integer, so the
+result cannot depend on the order of k) and on random data against a
+float64 torch reference (normwise, by dtype), over edge shapes that leave
+partial tiles and partial K slices."""
+import numpy as np
+import pytest
+import torch
+
+from cases import make_inputs
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+N_CFG = 22  # FM_TILE_NUM
+
+
+def _dev(a, fm):
+    return fm.col_major(torch.from_numpy(np.asfortranarray(a)).to(DEV))
+
+
+@pytest.fixture(autouse=True)
+def _reset(fm):
+    yield
+    fm.set_gemm_tile(-1)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("cfg", range(N_CFG))
+def test_tile_config_exact_on_integers(fm, cfg, dtype):
+    fm.set_gemm_tile(cfg)
+    for shape in [(1, 1, 1), (17, 33, 65), (128, 128, 128), (200, 300, 150), (257, 129, 383),
+                  (64, 1000, 48)]:
+        m, k, n = shape
+        A, B, C0 = make_inputs(m, k, n, 31, "int", dtype, with_c=True)
+        exact = A.astype(np.float64) @ B.astype(np.float64)
+        C = fm.gemm(_dev(A, fm), _dev(B, fm), _dev(C0, fm), 2.0, -3.0).cpu().numpy()
+        assert np.array_equal(C.astype(np.float64), 2 * exact - 3 * C0.astype(np.float64)), shape
+
+
+@pytest.mark.parametrize("dtype,tol", [(np.float64, 4e-16), (np.float32, 2e-7)])
+@pytest.mark.parametrize("cfg", range(N_CFG))
+def test_tile_config_random(fm, cfg, dtype, tol):
+    fm.set_gemm_tile(cfg)
+    m, k, n = 515, 1031, 263
+    A, B, _ = make_inputs(m, k, n, 32, "u11", dtype)
+    C = fm.gemm(_dev(A, fm), _dev(B, fm)).double().cpu().numpy()
+    ref = A.astype(np.float64) @ B.astype(np.float64)
+    err = np.abs(C - ref).max() / (np.abs(A).max() * np.abs(B).max() * k)
+    assert err < tol, err

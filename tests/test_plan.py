@@ -1,0 +1,80 @@
+"""Plan structure on the host (no GPU): two-level stage pairing, depth-first
+schedules and the leaf remainder split, through fm_plan_query.
+
+The expected counts follow from the recursion (algorithms/strassen.hpp:
+378-659) and the schedule rules: R^l nodes per level, levels paired from
+the deepest up when the middle level has no peeling, one child sub-plan per
+depth-first node, and the last part-empty round of leaf tiles moved to a
+second launch."""
+import pytest
+import torch
+
+import fastmm
+
+
+@pytest.fixture(autouse=True)
+def _reset():
+    yield
+    fastmm.set_schedule(0, 0, True)
+    fastmm.set_stage_fusion(True)
+
+
+def test_bench_workload_plan():
+    # grey424_26_206, 2 levels, 8192^3: one two-level S and T launch, W per
+    # level, 676 leaves of 512 x 2048 x 512 = 5408 tiles of 256 x 128 on
+    # 256 CUs: 21 full rounds (672 products) + 4 products on 64 x 64 tiles
+    p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
+    assert p["launches"] == {"S": 1, "T": 1, "W": 2, "LEAF": 2, "PEEL": 0}
+    assert p["two_level_stages"] == 2
+    assert p["leaf"] == [{"cfg": 7, "products": 672, "tiles": 5376},
+                         {"cfg": 1, "products": 4, "tiles": 256}]
+    fastmm.set_schedule(0, 0, False)
+    p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
+    assert p["leaf"] == [{"cfg": 7, "products": 676, "tiles": 5408}]
+
+
+def test_one_level_stages_when_middle_level_peels():
+    p = fastmm.plan_info("strassen", 130, 130, 130, 2)
+    assert p["two_level_stages"] == 0
+    assert p["launches"]["S"] == 2 and p["launches"]["T"] == 2
+    assert p["launches"]["PEEL"] == 1   # 130 splits evenly, 65 does not: level 1 peels
+    fastmm.set_stage_fusion(False)
+    p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
+    assert p["two_level_stages"] == 0 and p["launches"]["S"] == 2
+
+
+def test_depth_first_top_level():
+    fastmm.set_schedule(1)
+    p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
+    # level-0 S/T/W plus, per child, its own S/T/W and one leaf launch
+    assert p["launches"] == {"S": 27, "T": 27, "W": 27, "LEAF": 26, "PEEL": 0}
+    assert p["leaf"][0] == {"cfg": 7, "products": 26, "tiles": 208}
+    # level-0 S/T/M blocks plus one child's S/T/M blocks, 8-byte elements
+    lvl0 = 22 * 2048 * 4096 + 20 * 4096 * 2048 + 26 * 2048 * 2048
+    child = 22 * 512 * 2048 + 20 * 2048 * 512 + 26 * 512 * 512
+    assert p["workspace_bytes"] == 8 * (lvl0 + child)
+
+
+def test_workspace_limit_selects_depth_first():
+    bfs = fastmm.plan_info("strassen", 256, 256, 256, 2)["workspace_bytes"]
+    fastmm.set_schedule(1)
+    dfs = fastmm.plan_info("strassen", 256, 256, 256, 2)["workspace_bytes"]
+    # level 0: 5 S + 5 T + 7 M blocks of 128^2; one child: 5 + 5 + 7 of 64^2
+    assert dfs == 8 * (17 * 128 * 128 + 17 * 64 * 64) < bfs
+    fastmm.set_schedule(0, bfs)
+    assert fastmm.plan_info("strassen", 256, 256, 256, 2)["workspace_bytes"] == bfs
+    fastmm.set_schedule(0, bfs - 1)
+    assert fastmm.plan_info("strassen", 256, 256, 256, 2)["workspace_bytes"] == dfs
+    fastmm.set_schedule(0, 1000)
+    with pytest.raises(fastmm.FastMMError):
+        fastmm.plan_info("strassen", 256, 256, 256, 2)
+    with pytest.raises(ValueError):
+        fastmm.set_schedule(-1)
+
+
+def test_remainder_split_modes():
+    p = fastmm.plan_info("strassen", 4096, 4096, 4096, 2, dtype=torch.float32, mode="split_bf16")
+    assert [l["products"] for l in p["leaf"]] == [48, 1]
+    # int8-digit leaves keep one launch
+    p = fastmm.plan_info("strassen", 4096, 4096, 4096, 2, mode="ozaki_int8")
+    assert p["launches"]["LEAF"] == 1 and p["leaf"][0]["cfg"] == -1
