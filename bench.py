@@ -29,21 +29,27 @@ sys.path.insert(0, os.path.join(ROOT, "fast-matmul_amd"))
 
 PEAK_FP64_TF = 78.6
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per leaf launch of the default workload, from the two rocprofv3
-# --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in this file, scaled by a
-# 1 GiB copy kernel measured in the same runs
+# HBM bytes of the default workload's leaf launches per step, from the two
+# rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) summarised in this file,
+# scaled by a 1 GiB copy kernel measured in the same runs; valid for the leaf
+# plan recorded beside it
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_grey424_26_206_fp64_8192.md")
 
 
-def measured_traffic():
+def measured_traffic(leaf_launches):
+    """Leaf bytes per step from TRAFFIC_FILE if it was measured on this plan."""
+    plan = "+".join("%dx%d" % (l["cfg"], l["products"]) for l in leaf_launches)
+    val, rec = None, None
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch (leaf GEMM):"):
-                    return float(line.split(":")[1])
+                if line.startswith("traffic_bytes_per_step (leaf GEMM):"):
+                    val = float(line.split(":")[1])
+                if line.startswith("leaf plan (tile config x products):"):
+                    rec = line.split(":")[1].strip()
     except OSError:
-        pass
-    return None
+        return None
+    return val if rec == plan else None
 
 WORKLOADS = {
     # name: (alg, steps, M, K, N)
@@ -270,8 +276,18 @@ def main():
     leaf_ms_step = leaf_ms / args.steps
     achieved = F_leaf / (leaf_ms_step * 1e-3) / 1e12 if leaf_n else None
     plan = fastmm.plan_info(alg, M, K, N, steps)
-    stages = {k: {"ms_per_step": v[0] / args.steps, "launches_per_step": v[1] / args.steps}
-              for k, v in prof.items()}
+    # stage rates against HBM: algorithmic bytes of each stage (every source
+    # element read once, every output written once) over its measured time
+    plan_bytes = fastmm.plan_info(alg, M, K, N, steps)["stage_bytes"]
+    stages = {}
+    for k, v in prof.items():
+        ms = v[0] / args.steps
+        st = {"ms_per_step": ms, "launches_per_step": v[1] / args.steps}
+        if k in ("S", "T", "W") and ms > 0:
+            gbs = plan_bytes[k] / (ms * 1e-3) / 1e9
+            st.update({"algorithmic_GB": plan_bytes[k] / 1e9, "GB_per_s": gbs,
+                       "hbm_frac": gbs / PEAK_HBM_GBS})
+        stages[k] = st
 
     if args.copy_probe:
         x = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)
@@ -331,7 +347,7 @@ def main():
                      "flops_per_step": F_leaf, "leaf_ms_per_step": leaf_ms_step,
                      "leaf_launches_per_step": leaf_n / args.steps,
                      "leaf_launches": plan["leaf"],
-                     "traffic": measured_traffic() if args.workload == "c2sq" else None,
+                     "traffic": measured_traffic(plan["leaf"]) if args.workload == "c2sq" else None,
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)
                      if args.workload == "c2sq" else None},
         "cpu_baseline": cpu,

@@ -191,6 +191,44 @@ fm_status timed_run(const Plan& P, const void* A, const void* B, void* C, void* 
 }
 
 // Workspace for a plan: the caller's buffer if given, else the arena.
+// Algorithmic HBM bytes of one launch: every source element read once,
+// every output written once (the minimum any schedule of that launch needs).
+double launch_bytes(const Launch& L, double es) {
+  const FmAlgDesc* g = L.alg;
+  const double nodes = L.st.nodes;
+  switch (L.kind) {
+    case LK_S:
+      return es * nodes * L.st.rows * L.st.cols * (g->m * g->k + g->ns);
+    case LK_T:
+      return es * nodes * L.st.rows * L.st.cols * (g->k * g->n + g->nt);
+    case LK_S2:
+    case LK_T2: {
+      const bool a = L.kind == LK_S2;
+      const int nb = a ? g->m * g->k : g->k * g->n;
+      int mat = 0;
+      for (int r = 0; r < g->R; ++r) mat += a ? g->a_kind[r] : g->b_kind[r];
+      const double outs = (double)g->R * (a ? g->ns : g->nt) + (double)mat * (a ? g->a_nviews : g->b_nviews);
+      return es * nodes * L.st.rows2 * L.st.cols2 * ((double)nb * nb + outs);
+    }
+    case LK_W:
+      return es * nodes * L.st.rows * L.st.cols *
+             (g->R + g->m * g->n * (L.st.beta != 0.0 ? 2.0 : 1.0));
+    case LK_GEMM: {
+      double b = 0;
+      for (const FmGemmDesc& d : L.gdescs)
+        b += es * ((double)d.M * d.K + (double)d.K * d.N + (double)d.M * d.N * (d.beta != 0.0 ? 2 : 1));
+      return b;
+    }
+    case LK_OZAKI: {
+      double b = 0;
+      for (const FmOzakiDesc& d : L.odescs)
+        b += es * ((double)d.M * d.K + (double)d.K * d.N + (double)d.M * d.N * (d.beta != 0.0 ? 2 : 1));
+      return b;
+    }
+  }
+  return 0.0;
+}
+
 fm_status plan_workspace(const Plan& P, void* workspace, size_t workspace_bytes, void** ws) {
   *ws = workspace;
   if (P.ws_bytes == 0) return FM_OK;
@@ -265,8 +303,10 @@ fm_status fm_plan_query(const char* alg_name, fm_dtype dtype, fm_mode mode, int 
   memset(info, 0, sizeof(*info));
   info->workspace_bytes = P.ws_bytes;
   int nleaf = 0;
+  const double es = elem_size(dtype);
   for (const Launch& L : P.launches) {
     info->launches[L.stage] += 1;
+    info->stage_bytes[L.stage] += launch_bytes(L, es);
     if (L.kind == LK_S2 || L.kind == LK_T2) info->two_level_stages += 1;
     if (L.stage == FM_STAGE_LEAF) {
       const int tiles = L.kind == LK_OZAKI ? L.oz.total_tiles : L.gm.total_tiles;

@@ -57,7 +57,12 @@ __device__ __forceinline__ int fm_xcd_remap(int wg, int nwg) {
 // barrier wait is covered by the other's MFMAs).  UNR: unroll of the
 // 4-deep K steps inside one LDS slice (less unrolling, fewer live operand
 // registers).
-template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW = 1, int UNR = BK / 4>
+// PIPE: software-pipelined main loop (see the loop below).  FULL: every
+// product is a whole number of tiles and K slices with 16-byte aligned
+// columns (the launch checks), so only the unpredicated 16-byte load path
+// is compiled.
+template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW = 1, int UNR = BK / 4,
+          bool PIPE = false, bool FULL = false>
 __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs args) {
   using Op = FmMfma<T>;
   using acc_t = typename Op::acc_t;
@@ -120,7 +125,8 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   // Interior tiles of operands with 16-byte aligned columns take unpredicated
   // 16-byte loads for every full K slice; everything else takes the
   // predicated element loads.
-  const bool interior = CAN_VEC && args.vec && (m0 + BM <= M) && (n0 + BN <= N);
+  static_assert(!FULL || CAN_VEC, "full-tile variant needs the vector load mapping");
+  const bool interior = FULL || (CAN_VEC && args.vec && (m0 + BM <= M) && (n0 + BN <= N));
 
   // element-load coordinates
   const int a_row = tid % BM, a_k = tid / BM;
@@ -172,6 +178,7 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
       for (int e = 0; e < VL; ++e) Bs[buf][bv_col + i * BV_NSTEP][bv_k + e] = rb[i * VL + e];
   };
   auto gload = [&](int k0) {
+    if constexpr (FULL) return;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) {
       const int gk = k0 + a_k + i * A_KSTEP;
@@ -185,6 +192,7 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
     }
   };
   auto sstore = [&](int buf) {
+    if constexpr (FULL) return;
 #pragma unroll
     for (int i = 0; i < A_PER; ++i) As[buf][a_k + i * A_KSTEP][a_row] = ra[i];
 #pragma unroll
@@ -201,6 +209,60 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
     sstore(0);
   }
   __syncthreads();
+  if constexpr (PIPE) {
+    // Fragments of K step s + 1 are read from LDS before the MFMAs of step s
+    // (two register sets), and the MFMAs of a slice's last step are issued
+    // after the barrier that publishes the next slice, behind that slice's
+    // first fragment reads: no MFMA waits on an LDS read or a barrier unless
+    // the reads take longer than a step's MFMAs.
+    constexpr int NS4 = BK / 4;
+    T af[2][TM], bf[2][TN];
+    auto frag = [&](int buf, int k4, int set) {
+      const int kidx = k4 * 4 + (lane >> 4);
+#pragma unroll
+      for (int i = 0; i < TM; ++i) af[set][i] = As[buf][kidx][wm * WTM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bf[set][j] = Bs[buf][wn * WTN + j * 16 + (lane & 15)][kidx];
+    };
+    auto mma_step = [&](int set) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = Op::mma(bf[set][j], af[set][i], acc[i][j]);
+    };
+    if (nk > 1) {
+      if (interior && 2 * BK <= K)
+        gload_vec(BK);
+      else
+        gload(BK);
+    }
+    frag(0, 0, 0);
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+#pragma unroll
+      for (int k4 = 0; k4 < NS4 - 1; ++k4) {
+        frag(cur, k4 + 1, (k4 + 1) & 1);
+        mma_step(k4 & 1);
+      }
+      // the last step's fragments are in registers: publish the next slice
+      if (kt + 1 < nk) {
+        if (interior && (kt + 2) * BK <= K)
+          sstore_vec(cur ^ 1);
+        else
+          sstore(cur ^ 1);
+      }
+      __syncthreads();
+      if (kt + 2 < nk) {
+        if (interior && (kt + 3) * BK <= K)
+          gload_vec((kt + 2) * BK);
+        else
+          gload((kt + 2) * BK);
+      }
+      if (kt + 1 < nk) frag(cur ^ 1, 0, NS4 & 1);
+      __builtin_amdgcn_sched_barrier(0);
+      mma_step((NS4 - 1) & 1);
+    }
+  } else
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
     const bool more = kt + 1 < nk;
@@ -255,11 +317,23 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   }  // tile loop
 }
 
-template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW = 1, int UNR = BK / 4>
+template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW = 1, int UNR = BK / 4,
+          bool PIPE = false>
 static void fm_gemm_go(const FmGemmArgs& a, hipStream_t s) {
   const int grid = a.grid > 0 && a.grid < a.total_tiles ? a.grid : a.total_tiles;
-  hipLaunchKernelGGL((fm_gemm_kernel<T, BM, BN, BK, WM, WN, MINW, UNR>), dim3(grid),
-                     dim3(64 * WM * WN), 0, s, a);
+  constexpr int NT = 64 * WM * WN, VL = 16 / sizeof(T);
+  constexpr bool can_full = (BM * BK / NT) % VL == 0 && (BK * BN / NT) % VL == 0 &&
+                            BM % VL == 0 && BK % VL == 0 && (NT * VL) % BM == 0 &&
+                            (NT * VL) % BK == 0;
+  if constexpr (can_full) {
+    if (a.full && a.vec) {
+      hipLaunchKernelGGL((fm_gemm_kernel<T, BM, BN, BK, WM, WN, MINW, UNR, PIPE, true>),
+                         dim3(grid), dim3(NT), 0, s, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((fm_gemm_kernel<T, BM, BN, BK, WM, WN, MINW, UNR, PIPE, false>), dim3(grid),
+                     dim3(NT), 0, s, a);
 }
 
 template <typename T>
@@ -284,17 +358,19 @@ static hipError_t fm_gemm_launch(int cfg, const FmGemmArgs& a, hipStream_t s) {
     case FM_TILE_256x128_W8_U1: fm_gemm_go<T, 256, 128, 16, 4, 2, 1, 1>(a, s); break;
     case FM_TILE_128x128_O2_U1: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 1>(a, s); break;
     case FM_TILE_128x256_W8_U1: fm_gemm_go<T, 128, 256, 16, 2, 4, 1, 1>(a, s); break;
+    case FM_TILE_PP_256x128_W8: fm_gemm_go<T, 256, 128, 16, 4, 2, 1, 4, true>(a, s); break;
+    case FM_TILE_PP_128x256_W8: fm_gemm_go<T, 128, 256, 16, 2, 4, 1, 4, true>(a, s); break;
+    case FM_TILE_PP_128x128_W8: fm_gemm_go<T, 128, 128, 16, 2, 4, 1, 4, true>(a, s); break;
+    case FM_TILE_PP_128x128_O2: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 4, true>(a, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
 hipError_t fm_gemm_launch_f64(int cfg, const FmGemmArgs& a, hipStream_t s) {
-  if (cfg >= FM_TILE_DV_128x128) return fm_gemm_dv_launch_f64(cfg, a, s);
   return fm_gemm_launch<double>(cfg, a, s);
 }
 
 hipError_t fm_gemm_launch_f32(int cfg, const FmGemmArgs& a, hipStream_t s) {
-  if (cfg >= FM_TILE_DV_128x128) return fm_gemm_dv_launch_f32(cfg, a, s);
   return fm_gemm_launch<float>(cfg, a, s);
 }

@@ -25,7 +25,7 @@ struct FmGemmArgs {
   int32_t uniform;  // all products have one shape: tile -> product by division
   int32_t vec;      // every A/B column start is 16-byte aligned: vector loads allowed
   int32_t grid;     // > 0: persistent launch with this many work groups
-  int32_t pad;
+  int32_t full;     // every product is a whole number of tiles and K slices
 };
 
 // Tile configurations of the leaf kernel: BM x BN output tile, BK-deep K
@@ -49,13 +49,21 @@ enum FmTileCfg : int {
   FM_TILE_256x128_W8_U1 = 15,  // BK 16, 4x2 waves, K steps not unrolled
   FM_TILE_128x128_O2_U1 = 16,  // as 14, K steps not unrolled
   FM_TILE_128x256_W8_U1 = 17,  // BK 16, 2x4 waves, K steps not unrolled
-  // B straight to registers, only A through LDS (csrc/gemm_dv.hip)
-  FM_TILE_DV_128x128 = 18,     // 2x2 waves (64 x 64 each), two work groups per CU
-  FM_TILE_DV_128x128_W8 = 19,  // 2x4 waves (64 x 32 each)
-  FM_TILE_DV_256x128_W16 = 20, // 4x4 waves (64 x 32 each)
-  FM_TILE_DV_128x64 = 21,      // 2x2 waves (64 x 32 each), two work groups per CU
+  // software-pipelined main loop: LDS fragments double-buffered, the last
+  // K step of a slice issued after the barrier
+  FM_TILE_PP_256x128_W8 = 18,  // BK 16, 4x2 waves
+  FM_TILE_PP_128x256_W8 = 19,  // BK 16, 2x4 waves
+  FM_TILE_PP_128x128_W8 = 20,  // BK 16, 2x4 waves
+  FM_TILE_PP_128x128_O2 = 21,  // BK 16, 2x2 waves, registers for 2 waves per SIMD
   FM_TILE_NUM = 22
 };
+
+// K slice depth of each configuration
+inline int fm_tile_bk(int cfg) {
+  return (cfg == FM_TILE_128x128_K32 || cfg == FM_TILE_128x128_K32_W8 || cfg == FM_TILE_64x64_K32)
+             ? 32
+             : 16;
+}
 
 // output tile of each configuration
 inline void fm_tile_dims(int cfg, int* bm, int* bn) {
@@ -63,14 +71,12 @@ inline void fm_tile_dims(int cfg, int* bm, int* bn) {
                                         {128, 128}, {128, 128}, {128, 128}, {256, 128},
                                         {128, 256}, {256, 256}, {64, 64},  {128, 128},
                                         {128, 64},  {64, 128},  {128, 128}, {256, 128},
-                                        {128, 128}, {128, 256}, {128, 128}, {128, 128},
-                                        {256, 128}, {128, 64}};
+                                        {128, 128}, {128, 256}, {256, 128}, {128, 256},
+                                        {128, 128}, {128, 128}};
   *bm = d[cfg][0];
   *bn = d[cfg][1];
 }
 
-hipError_t fm_gemm_dv_launch_f64(int cfg, const FmGemmArgs& a, hipStream_t s);
-hipError_t fm_gemm_dv_launch_f32(int cfg, const FmGemmArgs& a, hipStream_t s);
 hipError_t fm_gemm_launch_f64(int cfg, const FmGemmArgs& a, hipStream_t s);
 hipError_t fm_gemm_launch_f32(int cfg, const FmGemmArgs& a, hipStream_t s);
 // FP32 data, split-bf16 slice products (csrc/gemm_split.hip)

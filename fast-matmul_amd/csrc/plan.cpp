@@ -157,10 +157,14 @@ Launch gemm_launch(std::vector<FmGemmDesc> descs, int cfg, int stage, fm_dtype d
   // multiple-of-4 (FP32) offsets and leading dimensions; bases are checked
   // at launch
   const int vl = dt == FM_F64 ? 2 : 4;
-  bool vec = true;
-  for (const FmGemmDesc& d : descs)
+  const int bk = fm_tile_bk(cfg);
+  bool vec = true, full = true;
+  for (const FmGemmDesc& d : descs) {
     vec = vec && d.a.off % vl == 0 && d.a.ld % vl == 0 && d.b.off % vl == 0 && d.b.ld % vl == 0;
+    full = full && d.M % bm == 0 && d.N % bn == 0 && d.K % bk == 0 && d.K > 0;
+  }
   L.vec_ok = vec ? 1 : 0;
+  L.gm.full = full ? 1 : 0;
   L.cfg = cfg;
   L.gdescs = std::move(descs);
   return L;
@@ -242,30 +246,50 @@ int elem_size(fm_dtype dt) { return dt == FM_F64 ? 8 : 4; }
 // 8192^3) and its resident work groups per CU set how many rounds of tiles
 // the launch takes.
 namespace {
+// A tile shape with the kernel used when every product of the launch is a
+// whole number of tiles and K slices with 16-byte aligned columns
+// (cfg_full), and the kernel for launches with edges (cfg_edge; -1: the
+// shape is not offered, its edge kernel spills registers).
 struct TileCand {
-  int cfg;
+  int cfg_full, cfg_edge;
   double eff;  // throughput per tile area, relative
   int per_cu;  // resident work groups per CU
 };
-const TileCand kF64Tiles[] = {{FM_TILE_256x128_W8, 1.00, 1}, {FM_TILE_128x256_W8, 0.99, 1},
-                              {FM_TILE_128x128_W16, 0.93, 2}, {FM_TILE_128x128_W8, 0.90, 2},
-                              {FM_TILE_128x64_W8, 0.91, 3},  {FM_TILE_64x64, 0.80, 4}};
-const TileCand kF32Tiles[] = {{FM_TILE_256x256_W16, 1.00, 1}, {FM_TILE_256x128_W8, 0.90, 2},
-                              {FM_TILE_128x256_W8, 0.90, 2},  {FM_TILE_128x128_W8, 0.82, 4},
-                              {FM_TILE_64x128, 0.77, 4},      {FM_TILE_128x64, 0.74, 4},
-                              {FM_TILE_64x64, 0.69, 8}};
-const TileCand kSplitTiles[] = {{FM_TILE_128x128, 1.00, 2}, {FM_TILE_64x64, 0.80, 4}};
+// Relative throughput: whole-tile rates on MI355X (8192^3 and the 676-leaf
+// product of 512 x 2048 x 512), best = 1.  Work groups per CU: the lesser of
+// what the register allocation (waves per SIMD x 4 / waves per group) and
+// the 160 KB of LDS allow.
+const TileCand kF64Tiles[] = {
+    {FM_TILE_PP_128x128_O2, FM_TILE_128x128_O2_U1, 1.00, 2},
+    {FM_TILE_128x256_W8, FM_TILE_128x256_W8_U1, 0.99, 1},
+    {FM_TILE_256x128_W8, FM_TILE_256x128_W8_U1, 0.975, 1},
+    {FM_TILE_128x128_W16, FM_TILE_128x128_W16, 0.92, 1},
+    {FM_TILE_128x64_W8, FM_TILE_128x64_W8, 0.90, 2},
+    {FM_TILE_64x64, FM_TILE_64x64, 0.80, 3}};
+const TileCand kF32Tiles[] = {
+    {FM_TILE_256x256_W16, -1, 1.00, 1},
+    {FM_TILE_PP_128x128_W8, FM_TILE_PP_128x128_W8, 0.995, 1},
+    {FM_TILE_PP_128x256_W8, FM_TILE_PP_128x256_W8, 0.96, 1},
+    {FM_TILE_256x128_W8, FM_TILE_256x128_W8, 0.95, 1},
+    {FM_TILE_128x128_W8, FM_TILE_128x128_W8, 0.79, 2},
+    {FM_TILE_64x128, FM_TILE_64x128, 0.74, 3},
+    {FM_TILE_128x64, FM_TILE_128x64, 0.71, 3},
+    {FM_TILE_64x64, FM_TILE_64x64, 0.66, 4}};
+const TileCand kSplitTiles[] = {{FM_TILE_128x128, FM_TILE_128x128, 1.00, 2},
+                                {FM_TILE_64x64, FM_TILE_64x64, 0.80, 4}};
+// edge kernels run slower than whole-tile ones (predicated loads)
+const double kEdgePenalty = 0.93;
 
 const TileCand* tile_cands(fm_dtype dt, fm_mode mode, int* n) {
   if (mode == FM_MODE_SPLIT_BF16) {
-    *n = 2;
+    *n = (int)(sizeof(kSplitTiles) / sizeof(kSplitTiles[0]));
     return kSplitTiles;
   }
   if (dt == FM_F64) {
-    *n = 6;
+    *n = (int)(sizeof(kF64Tiles) / sizeof(kF64Tiles[0]));
     return kF64Tiles;
   }
-  *n = 7;
+  *n = (int)(sizeof(kF32Tiles) / sizeof(kF32Tiles[0]));
   return kF32Tiles;
 }
 
@@ -278,22 +302,54 @@ int64_t count_tiles(int cfg, const std::vector<FmGemmDesc>& prods, size_t from =
   return tiles;
 }
 
+// Every product a whole number of tiles and K slices, columns 16-byte aligned.
+bool whole_tiles(fm_dtype dt, int cfg, const std::vector<FmGemmDesc>& prods) {
+  int bm, bn;
+  fm_tile_dims(cfg, &bm, &bn);
+  const int bk = fm_tile_bk(cfg), vl = dt == FM_F64 ? 2 : 4;
+  for (const FmGemmDesc& d : prods)
+    if (d.M % bm || d.N % bn || d.K % bk || d.K == 0 || d.a.off % vl || d.a.ld % vl ||
+        d.b.off % vl || d.b.ld % vl)
+      return false;
+  return true;
+}
+
 // Estimated time of one launch of `tiles` tiles: rounds of per_cu work
 // groups on each of the 256 CUs.
-double tile_time(const TileCand& c, int64_t tiles) {
+double tile_time(const TileCand& c, int64_t tiles, bool full) {
   int bm, bn;
-  fm_tile_dims(c.cfg, &bm, &bn);
+  fm_tile_dims(c.cfg_full, &bm, &bn);
   const int64_t slots = 256LL * c.per_cu;
   const double rounds = (double)((tiles + slots - 1) / slots);
-  return rounds * bm * bn * c.per_cu / c.eff;
+  return rounds * bm * bn * c.per_cu / (c.eff * (full ? 1.0 : kEdgePenalty));
 }
 
 const TileCand* find_cand(fm_dtype dt, fm_mode mode, int cfg) {
   int n;
   const TileCand* c = tile_cands(dt, mode, &n);
   for (int i = 0; i < n; ++i)
-    if (c[i].cfg == cfg) return &c[i];
+    if (c[i].cfg_full == cfg || c[i].cfg_edge == cfg) return &c[i];
   return nullptr;
+}
+
+// Best candidate for a product list: its kernel and estimated time.
+int best_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods, double* t_out) {
+  int nc;
+  const TileCand* cands = tile_cands(dt, mode, &nc);
+  int best = FM_TILE_64x64;
+  double best_t = 1e300;
+  for (int i = 0; i < nc; ++i) {
+    const bool full = whole_tiles(dt, cands[i].cfg_full, prods);
+    const int cfg = full ? cands[i].cfg_full : cands[i].cfg_edge;
+    if (cfg < 0) continue;
+    const double t = tile_time(cands[i], count_tiles(cfg, prods), full);
+    if (t < best_t * 0.999) {
+      best_t = t;
+      best = cfg;
+    }
+  }
+  if (t_out) *t_out = best_t;
+  return best;
 }
 }  // namespace
 
@@ -301,18 +357,7 @@ int g_split_remainder = 1;
 
 int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods) {
   if (g_tile_override >= 0) return g_tile_override;
-  int nc;
-  const TileCand* cands = tile_cands(dt, mode, &nc);
-  int best = FM_TILE_64x64;
-  double best_t = 1e300;
-  for (int i = 0; i < nc; ++i) {
-    const double t = tile_time(cands[i], count_tiles(cands[i].cfg, prods));
-    if (t < best_t * 0.999) {
-      best_t = t;
-      best = cands[i].cfg;
-    }
-  }
-  return best;
+  return best_tile(dt, mode, prods, nullptr);
 }
 
 // The GPU form of the reference's HYBRID remainder rule (util/par_util.hpp:
@@ -322,7 +367,7 @@ int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods) {
 // launch with the tile that finishes them soonest.  Products stay whole, so
 // every output element is computed by the same instructions either way.
 void split_remainder(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods,
-                     std::vector<FmGemmDesc>& rest, int* cfg_rest) {
+                     std::vector<FmGemmDesc>& rest, int* cfg_rest, int* cfg_head) {
   if (g_tile_override >= 0 || P.mode == FM_MODE_OZAKI_INT8) return;
   const TileCand* c = find_cand(P.dtype, P.mode, cfg);
   if (!c) return;
@@ -343,15 +388,19 @@ void split_remainder(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods,
   }
   if (acc != full || cut == prods.size()) return;
   std::vector<FmGemmDesc> tail(prods.begin() + (std::ptrdiff_t)cut, prods.end());
-  const int cr = pick_tile(P.dtype, P.mode, tail);
-  const TileCand* ct = find_cand(P.dtype, P.mode, cr);
-  if (!ct) return;
-  const double t_one = tile_time(*c, total);
-  const double t_two = tile_time(*c, full) + tile_time(*ct, count_tiles(cr, tail));
+  std::vector<FmGemmDesc> head(prods.begin(), prods.begin() + (std::ptrdiff_t)cut);
+  double t_tail = 0;
+  const int cr = best_tile(P.dtype, P.mode, tail, &t_tail);
+  const bool whole = whole_tiles(P.dtype, c->cfg_full, prods);
+  const double t_one = tile_time(*c, total, whole);
+  const double t_two = tile_time(*c, full, whole_tiles(P.dtype, c->cfg_full, head)) + t_tail;
   if (t_two >= t_one * 0.99) return;
   prods.resize(cut);
   rest = std::move(tail);
   *cfg_rest = cr;
+  // the full rounds may be whole tiles even where the remainder was not
+  if (c->cfg_edge >= 0 || whole_tiles(P.dtype, c->cfg_full, prods))
+    *cfg_head = whole_tiles(P.dtype, c->cfg_full, prods) ? c->cfg_full : c->cfg_edge;
 }
 
 // Depth-first top level (the reference's DFS order, algorithms/strassen.hpp:
@@ -581,9 +630,10 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
     } else {
       const int cfg = pick_tile(P.dtype, P.mode, leaves);
       std::vector<FmGemmDesc> rest;
-      int cfg_rest = cfg;
-      if (persistent_grid == 0 && g_split_remainder) split_remainder(P, cfg, leaves, rest, &cfg_rest);
-      P.launches.push_back(gemm_launch(std::move(leaves), cfg, FM_STAGE_LEAF, P.dtype));
+      int cfg_rest = cfg, cfg_head = cfg;
+      if (persistent_grid == 0 && g_split_remainder)
+        split_remainder(P, cfg, leaves, rest, &cfg_rest, &cfg_head);
+      P.launches.push_back(gemm_launch(std::move(leaves), cfg_head, FM_STAGE_LEAF, P.dtype));
       P.launches.back().gm.grid = persistent_grid;
       if (!rest.empty()) P.launches.push_back(gemm_launch(std::move(rest), cfg_rest, FM_STAGE_LEAF, P.dtype));
     }

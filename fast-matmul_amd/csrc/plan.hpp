@@ -100,7 +100,7 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
 int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods);
 // move the products of a part-empty last round of tiles to `rest`
 void split_remainder(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods,
-                     std::vector<FmGemmDesc>& rest, int* cfg_rest);
+                     std::vector<FmGemmDesc>& rest, int* cfg_rest, int* cfg_head);
 extern int g_split_remainder;
 extern int g_dfs_levels;
 extern size_t g_ws_limit;

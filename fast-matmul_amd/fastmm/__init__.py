@@ -156,6 +156,7 @@ def plan_info(alg_name: str, m: int, k: int, n: int, num_steps: int = 1,
     check(LIB.fm_plan_query(alg_name.encode(), _DT[dtype], _lib.MODES[mode], int(m), int(k),
                             int(n), int(num_steps), ctypes.byref(info)))
     return {"launches": {nm: info.launches[i] for i, nm in enumerate(_lib.STAGES)},
+            "stage_bytes": {nm: info.stage_bytes[i] for i, nm in enumerate(_lib.STAGES)},
             "two_level_stages": info.two_level_stages,
             "leaf": [{"cfg": info.leaf_cfg[i], "products": info.leaf_products[i],
                       "tiles": info.leaf_tiles[i]} for i in range(min(2, info.launches[3]))],

@@ -38,7 +38,8 @@ STAGES = ("S", "T", "W", "LEAF", "PEEL")
 
 
 class FmPlanInfo(ctypes.Structure):
-    _fields_ = [("launches", ctypes.c_int * 5), ("two_level_stages", ctypes.c_int),
+    _fields_ = [("launches", ctypes.c_int * 5), ("stage_bytes", ctypes.c_double * 5),
+                ("two_level_stages", ctypes.c_int),
                 ("leaf_cfg", ctypes.c_int * 2), ("leaf_products", ctypes.c_int * 2),
                 ("leaf_tiles", ctypes.c_int * 2), ("workspace_bytes", ctypes.c_size_t)]
 

@@ -452,11 +452,92 @@ ENUM_TO_NAMESPACE: Dict[str, str] = {
 }
 
 
+# ---------------------------------------------------------------------------
+# Permuted algorithms (codegen/convert.py:61-99,136-156).  From <m,k,n;R>:
+#   transposition  C^T = B^T A^T  gives <n,k,m;R>:
+#       U'[(j,i)] = V[(i,j)],  V'[(j,i)] = U[(i,j)],  W'[(j,i)] = W[(i,j)];
+#   the cyclic symmetry of the multiplication tensor gives <n,m,k;R>:
+#       U'[(j,i)] = W[(i,j)],  V' = U,  W'[(j,i)] = V[(i,j)];
+# and compositions of the two give all six orders.  Coefficients are
+# unchanged; only the block numbering moves, so a permuted table is exact
+# iff its source is (checked by the Brent equations in tests/test_tables.py).
+# Tables with CSE groups are not permuted (their temporaries would change
+# sides); the registry derives only permutations absent from the committed
+# tables.
+# ---------------------------------------------------------------------------
+def _transposed_rows(rows, r: int, c: int):
+    """Rows of an r x c block grid (row-major) reordered to the c x r grid."""
+    if len(rows) != r * c:
+        raise ValueError("coefficient group has %d rows, expected %d" % (len(rows), r * c))
+    return [list(rows[i * c + j]) for j in range(c) for i in range(r)]
+
+
+def _swap(t: Table) -> Table:
+    m, k, n = t.m, t.k, t.n
+    return Table(name=t.name, m=n, k=k, n=m, U=_transposed_rows(t.V, k, n),
+                 V=_transposed_rows(t.U, m, k), W=_transposed_rows(t.W, m, n), path=t.path)
+
+
+def _cyclic(t: Table) -> Table:
+    m, k, n = t.m, t.k, t.n
+    return Table(name=t.name, m=n, k=m, n=k, U=_transposed_rows(t.W, m, n),
+                 V=[list(row) for row in t.U], W=_transposed_rows(t.V, k, n), path=t.path)
+
+
+def permute_table(t: Table, perm: Tuple[int, int, int], name: str = "") -> Table:
+    """The <d[p0], d[p1], d[p2]> algorithm from t = <d0, d1, d2> (perm as in
+    codegen/convert.py's s1,s2,s3 argument)."""
+    if t.SX or t.TX or t.MX:
+        raise ValueError("%s: tables with common-subexpression groups are not permuted" % t.name)
+    perm = tuple(perm)
+    if perm == (0, 1, 2):
+        out = Table(name=t.name, m=t.m, k=t.k, n=t.n, U=[list(r) for r in t.U],
+                    V=[list(r) for r in t.V], W=[list(r) for r in t.W], path=t.path)
+    elif perm == (2, 0, 1):
+        out = _cyclic(t)
+    elif perm == (1, 2, 0):
+        out = _cyclic(_cyclic(t))
+    elif perm == (2, 1, 0):
+        out = _swap(t)
+    elif perm == (0, 2, 1):
+        out = _swap(_cyclic(_cyclic(t)))
+    elif perm == (1, 0, 2):
+        out = _swap(_cyclic(t))
+    else:
+        raise ValueError("invalid permutation %r" % (perm,))
+    dims = (t.m, t.k, t.n)
+    assert (out.m, out.k, out.n) == (dims[perm[0]], dims[perm[1]], dims[perm[2]])
+    out.name = name or t.name
+    return out
+
+
+# derived name -> (source namespace, permutation)
+_DERIVED: Dict[str, Tuple[str, Tuple[int, int, int]]] = {
+    "grey232_11_50": ("grey322_11_50", (1, 0, 2)),
+    "grey223_11_50": ("grey322_11_50", (1, 2, 0)),
+    "grey242_14_84": ("grey422_14_84", (1, 0, 2)),
+    "grey224_14_84": ("grey422_14_84", (1, 2, 0)),
+    "grey225_18_99": ("grey522_18_99", (1, 2, 0)),
+    "grey244_26_257": ("grey424_26_257", (1, 0, 2)),
+    "grey334_29_234": ("grey343_29_234", (0, 2, 1)),
+    "grey233_15_103": ("grey323_15_103", (1, 0, 2)),
+    "hk233_15_94": ("hk323_15_94", (1, 0, 2)),
+    "tichavsky_kovac633_40_960": ("tichavsky_kovac336_40_960", (2, 1, 0)),
+}
+
+
+def derived_algorithms() -> Dict[str, Tuple[str, Tuple[int, int, int]]]:
+    return dict(_DERIVED)
+
+
 def algorithm_names() -> List[str]:
-    return sorted(_REGISTRY)
+    return sorted(list(_REGISTRY) + list(_DERIVED))
 
 
 def get_table(name: str) -> Table:
+    if name in _DERIVED:
+        src, perm = _DERIVED[name]
+        return permute_table(get_table(src), perm, name)
     if name not in _REGISTRY:
         raise KeyError("unknown algorithm %r" % name)
     rel, dims = _REGISTRY[name]

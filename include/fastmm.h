@@ -165,11 +165,14 @@ fm_status fm_set_ozaki_slices(int slices);
 fm_status fm_set_gemm_tile(int cfg);
 
 /* Shape of the plan fm_fastmatmul would run (built on the host, nothing
- * launched; contiguous operands): launches per fm_stage_kind, how many of
- * the S/T launches are two-level stages, and the first two leaf launches'
- * tile configuration (-1: int8-digit leaves), product and tile counts. */
+ * launched; contiguous operands): launches per fm_stage_kind, algorithmic
+ * HBM bytes per fm_stage_kind (every source element read once, every output
+ * written once), how many of the S/T launches are two-level stages, and the
+ * first two leaf launches' tile configuration (-1: int8-digit leaves),
+ * product and tile counts. */
 typedef struct fm_plan_info {
   int launches[5];
+  double stage_bytes[5];
   int two_level_stages;
   int leaf_cfg[2];
   int leaf_products[2];

@@ -29,8 +29,9 @@ def _reset(fm):
 @pytest.mark.parametrize("cfg", range(N_CFG))
 def test_tile_config_exact_on_integers(fm, cfg, dtype):
     fm.set_gemm_tile(cfg)
-    for shape in [(1, 1, 1), (17, 33, 65), (128, 128, 128), (200, 300, 150), (257, 129, 383),
-                  (64, 1000, 48)]:
+    # whole-tile shapes (every configuration's full-tile variant) and edges
+    for shape in [(1, 1, 1), (17, 33, 65), (128, 128, 128), (256, 256, 256), (512, 160, 512),
+                  (200, 300, 150), (257, 129, 383), (64, 1000, 48)]:
         m, k, n = shape
         A, B, C0 = make_inputs(m, k, n, 31, "int", dtype, with_c=True)
         exact = A.astype(np.float64) @ B.astype(np.float64)
@@ -38,11 +39,12 @@ def test_tile_config_exact_on_integers(fm, cfg, dtype):
         assert np.array_equal(C.astype(np.float64), 2 * exact - 3 * C0.astype(np.float64)), shape
 
 
+@pytest.mark.parametrize("shape", [(515, 1031, 263), (512, 1024, 512)])
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 4e-16), (np.float32, 2e-7)])
 @pytest.mark.parametrize("cfg", range(N_CFG))
-def test_tile_config_random(fm, cfg, dtype, tol):
+def test_tile_config_random(fm, cfg, dtype, tol, shape):
     fm.set_gemm_tile(cfg)
-    m, k, n = 515, 1031, 263
+    m, k, n = shape
     A, B, _ = make_inputs(m, k, n, 32, "u11", dtype)
     C = fm.gemm(_dev(A, fm), _dev(B, fm)).double().cpu().numpy()
     ref = A.astype(np.float64) @ B.astype(np.float64)

@@ -73,6 +73,10 @@ INT_CASES = [
     ("hk323_15_84", 2, (402, 500, 562)), ("grey522_18_99", 2, (1007, 400, 679)),
     ("smirnov336_40_960", 1, (402, 500, 562)), ("classical333_27_81", 1, (100, 100, 100)),
     ("grey252_18_99", 1, (64, 64, 64)), ("strassen", 0, (129, 65, 33)),
+    # permuted tables (codegen/convert.py), derived at build time
+    ("grey232_11_50", 2, (203, 305, 190)), ("grey224_14_84", 2, (130, 77, 321)),
+    ("grey244_26_257", 2, (130, 310, 290)), ("grey334_29_234", 1, (300, 301, 405)),
+    ("tichavsky_kovac633_40_960", 1, (610, 300, 310)),
 ]
 
 
@@ -91,6 +95,24 @@ def test_fast_exact_on_integers(fm, alg, steps, shape, dtype):
     C = _host(fm.fastmatmul(alg, Ad, Bd, _to_dev(C0, fm), steps, alpha=3.0, beta=-2.0))
     want = 3 * exact - (2 * C0.astype(np.float64) if steps > 0 else 0)
     assert np.array_equal(C.astype(np.float64), want)
+
+
+DERIVED = ["grey232_11_50", "grey223_11_50", "grey242_14_84", "grey224_14_84", "grey225_18_99",
+           "grey244_26_257", "grey334_29_234", "grey233_15_103", "hk233_15_94",
+           "tichavsky_kovac633_40_960"]
+
+
+@pytest.mark.parametrize("alg", DERIVED)
+def test_permuted_algorithms_vs_oracle(fm, oracle_mod, alg):
+    """Permuted tables against the oracle running the same table: normwise
+    within twice the oracle's own error against classical GEMM + 1e-15."""
+    A, B, C0 = make_inputs(301, 257, 333, 41, "u11", np.float64, with_c=True)
+    want = oracle_mod.fastmatmul(alg, A, B, C0, 2, 0.5, 2.0)
+    classical = 0.5 * (A @ B) + 2.0 * C0
+    got = _host(fm.fastmatmul(alg, _to_dev(A, fm), _to_dev(B, fm), _to_dev(C0, fm), 2,
+                              alpha=0.5, beta=2.0))
+    e_ref = _normwise(want, classical, A, B)
+    assert _normwise(got, want, A, B) <= 2 * e_ref + 1e-15
 
 
 def _golden_params():

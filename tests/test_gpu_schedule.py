@@ -94,8 +94,9 @@ def test_workspace_limit_with_caller_workspace(fm):
 
 
 def test_remainder_split_bit_identical(fm):
-    # 676 leaves of 256 x 1024 x 256: 1352 tiles of 256 x 128, i.e. 5 full
-    # rounds on 256 CUs (640 products) and 36 products in a second launch
+    # 676 leaves of 256 x 1024 x 256: 2704 tiles of 128 x 128 (one work group
+    # per CU), i.e. 10 full rounds on 256 CUs (640 products) and 36 products
+    # in a second launch
     info = fm.plan_info("grey424_26_206", 4096, 4096, 4096, 2)
     assert [l["products"] for l in info["leaf"]] == [640, 36]
     A, B, _ = make_inputs(4096, 4096, 4096, 24, "u11", np.float64)

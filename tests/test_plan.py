@@ -21,16 +21,17 @@ def _reset():
 
 def test_bench_workload_plan():
     # grey424_26_206, 2 levels, 8192^3: one two-level S and T launch, W per
-    # level, 676 leaves of 512 x 2048 x 512 = 5408 tiles of 256 x 128 on
-    # 256 CUs: 21 full rounds (672 products) + 4 products on 64 x 64 tiles
+    # level, 676 leaves of 512 x 2048 x 512 = 10816 tiles of 128 x 128
+    # (FmTileCfg 21, two work groups per CU) on 256 CUs: 21 full rounds
+    # (672 products) + 4 products on 64 x 64 tiles (FmTileCfg 1)
     p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
     assert p["launches"] == {"S": 1, "T": 1, "W": 2, "LEAF": 2, "PEEL": 0}
     assert p["two_level_stages"] == 2
-    assert p["leaf"] == [{"cfg": 7, "products": 672, "tiles": 5376},
+    assert p["leaf"] == [{"cfg": 21, "products": 672, "tiles": 10752},
                          {"cfg": 1, "products": 4, "tiles": 256}]
     fastmm.set_schedule(0, 0, False)
     p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
-    assert p["leaf"] == [{"cfg": 7, "products": 676, "tiles": 5408}]
+    assert p["leaf"] == [{"cfg": 21, "products": 676, "tiles": 10816}]
 
 
 def test_one_level_stages_when_middle_level_peels():
@@ -48,7 +49,7 @@ def test_depth_first_top_level():
     p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
     # level-0 S/T/W plus, per child, its own S/T/W and one leaf launch
     assert p["launches"] == {"S": 27, "T": 27, "W": 27, "LEAF": 26, "PEEL": 0}
-    assert p["leaf"][0] == {"cfg": 7, "products": 26, "tiles": 208}
+    assert p["leaf"][0] == {"cfg": 21, "products": 26, "tiles": 416}
     # level-0 S/T/M blocks plus one child's S/T/M blocks, 8-byte elements
     lvl0 = 22 * 2048 * 4096 + 20 * 4096 * 2048 + 26 * 2048 * 2048
     child = 22 * 512 * 2048 + 20 * 2048 * 512 + 26 * 512 * 512
@@ -78,3 +79,20 @@ def test_remainder_split_modes():
     # int8-digit leaves keep one launch
     p = fastmm.plan_info("strassen", 4096, 4096, 4096, 2, mode="ozaki_int8")
     assert p["launches"]["LEAF"] == 1 and p["leaf"][0]["cfg"] == -1
+
+
+def test_stage_bytes_match_closed_forms():
+    """Algorithmic bytes per stage, grey424_26_206 L=2 FP64 8192^3: one-level
+    S = 8 (blocks of level 1 + 26 x blocks of level 2) x (8 reads + 22 writes),
+    the per-level figures of the survey (8.56 / 7.99 / 3.70 GB); two-level S
+    reads A once and writes 26 x (22 outputs + 4 views of 22 products)."""
+    fastmm.set_stage_fusion(False)
+    b = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)["stage_bytes"]
+    assert b["S"] == 8 * 30 * (2048 * 4096 + 26 * 512 * 2048)
+    assert b["T"] == 8 * 28 * (4096 * 2048 + 26 * 2048 * 512)
+    assert b["W"] == 8 * (26 + 16) * (2048 * 2048 + 26 * 512 * 512)
+    assert b["LEAF"] == 8 * 676 * (512 * 2048 + 2048 * 512 + 512 * 512)
+    fastmm.set_stage_fusion(True)
+    b = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)["stage_bytes"]
+    assert b["S"] == 8 * 512 * 2048 * (64 + 26 * 22 + 22 * 4)
+    assert b["T"] == 8 * 2048 * 512 * (64 + 26 * 20 + 20 * 6)

@@ -34,7 +34,8 @@ def test_brent_apa_is_order_x(name):
 
 
 @pytest.mark.parametrize("name", [n for n in T.algorithm_names()
-                                  if not T._REGISTRY[n][0].startswith("eliminated/")])
+                                  if n in T.derived_algorithms()
+                                  or not T._REGISTRY[n][0].startswith("eliminated/")])
 def test_nnz_matches_file_name(name):
     t = T.get_table(name)
     nums = re.findall(r"\d+", os.path.basename(t.path))
@@ -42,6 +43,49 @@ def test_nnz_matches_file_name(name):
         assert t.nnz() == 36  # strassen: 12 nonzeros in each of U, V, W
         return
     assert str(t.nnz()) == nums[-1]
+    if name in T.derived_algorithms():  # the derived name carries the same count
+        assert name.endswith("_" + nums[-1])
+
+
+PERMS = [(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]
+
+
+@pytest.mark.parametrize("name", ["strassen", "grey322_11_50", "grey423_20_144", "grey424_26_257",
+                                  "smirnov333_23_139", "hk332_15_94", "grey522_18_99"])
+def test_all_six_permutations_are_exact(name):
+    """codegen/convert.py:61-99 derives every <.,.,.> order of a table; each
+    must satisfy the Brent equations exactly, with dims permuted and the
+    coefficient multiset unchanged."""
+    t = T.get_table(name)
+    d = (t.m, t.k, t.n)
+    for perm in PERMS:
+        p = T.permute_table(t, perm)
+        assert (p.m, p.k, p.n) == (d[perm[0]], d[perm[1]], d[perm[2]])
+        assert p.R == t.R and p.nnz() == t.nnz()
+        assert T.brent_residual(p) == 0, perm
+
+
+def test_permutation_group_laws():
+    """Transposition is an involution and the cyclic shift has order 3."""
+    t = T.get_table("grey423_20_144")
+    same = lambda a, b: ((a.m, a.k, a.n), a.U, a.V, a.W) == ((b.m, b.k, b.n), b.U, b.V, b.W)
+    assert same(T.permute_table(T.permute_table(t, (2, 1, 0)), (2, 1, 0)), t)
+    c = T.permute_table(t, (2, 0, 1))
+    c = T.permute_table(c, (2, 0, 1))
+    c = T.permute_table(c, (2, 0, 1))
+    assert same(c, t)
+
+
+def test_brent_check_catches_a_wrong_permutation():
+    """A plausible slip (transposing without moving W's block grid) breaks
+    the Brent equations, so the exactness check above has teeth."""
+    t = T.get_table("grey423_20_144")
+    good = T.permute_table(t, (2, 1, 0))
+    bad = T.Table(name="bad", m=good.m, k=good.k, n=good.n, U=good.U, V=good.V,
+                  W=[list(r) for r in t.W])   # W rows left in <4,2,3> order
+    assert T.brent_residual(bad) != 0
+    with pytest.raises(ValueError):
+        T.permute_table(T.get_table("grey424_26_206"), (1, 0, 2))  # CSE groups
 
 
 @pytest.mark.parametrize("elim,orig", [("grey424_26_206", "grey424_26_257"),
