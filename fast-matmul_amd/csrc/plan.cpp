@@ -268,7 +268,7 @@ const TileCand kF64Tiles[] = {
     {FM_TILE_64x64, FM_TILE_64x64, 0.80, 3}};
 const TileCand kF32Tiles[] = {
     {FM_TILE_256x256_W16, -1, 1.00, 1},
-    {FM_TILE_PP_128x128_W8, FM_TILE_PP_128x128_W8, 0.995, 1},
+    {FM_TILE_PP_128x128_W8, FM_TILE_128x128_W8, 0.995, 1},
     {FM_TILE_PP_128x256_W8, FM_TILE_PP_128x256_W8, 0.96, 1},
     {FM_TILE_256x128_W8, FM_TILE_256x128_W8, 0.95, 1},
     {FM_TILE_128x128_W8, FM_TILE_128x128_W8, 0.79, 2},

@@ -253,7 +253,7 @@ class Batch:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h:
+        if h and LIB is not None:  # module globals may be gone at interpreter exit
             LIB.fm_batch_destroy(h)
             self._h = None
 
