@@ -45,6 +45,9 @@ struct FmOp {
 // ws + out_off + ((q * R + r) * NOUT + o) * out_slab, and the sub-blocks of
 // a materialised child that its own children use unchanged (the "views")
 // go to ws + view_off + ((q * R + r) * NVIEW + slot) * out_slab.
+// Two-level W stage: ops[q] are the level-l C operands (blocks rows x cols,
+// sub-blocks rows2 x cols2); the level-(l+2) products of node q are at
+// ws + out_off + ((q * R + r) * R + r2) * out_slab.
 struct FmStageArgs {
   FmBases bases;
   const FmOp* ops;
@@ -79,6 +82,8 @@ struct FmAlgDesc {
   int a_nviews, b_nviews;
   const int* a_view_slot;
   const int* b_view_slot;
+  // two-level W stage (nullptr where a level's products do not fit in LDS)
+  FmStageLaunch w2_d, w2_f;
 };
 
 // generated registry (csrc/gen/registry.cpp): one accessor per algorithm

@@ -213,6 +213,13 @@ double launch_bytes(const Launch& L, double es) {
     case LK_W:
       return es * nodes * L.st.rows * L.st.cols *
              (g->R + g->m * g->n * (L.st.beta != 0.0 ? 2.0 : 1.0));
+    case LK_W2: {
+      const double mn = g->m * g->n;
+      return es * nodes * L.st.rows2 * L.st.cols2 *
+             ((double)g->R * g->R + mn * mn * (L.st.beta != 0.0 ? 2.0 : 1.0));
+    }
+    case LK_NONE:
+      return 0.0;
     case LK_GEMM: {
       double b = 0;
       for (const FmGemmDesc& d : L.gdescs)
@@ -307,7 +314,7 @@ fm_status fm_plan_query(const char* alg_name, fm_dtype dtype, fm_mode mode, int 
   for (const Launch& L : P.launches) {
     info->launches[L.stage] += 1;
     info->stage_bytes[L.stage] += launch_bytes(L, es);
-    if (L.kind == LK_S2 || L.kind == LK_T2) info->two_level_stages += 1;
+    if (L.kind == LK_S2 || L.kind == LK_T2 || L.kind == LK_W2) info->two_level_stages += 1;
     if (L.stage == FM_STAGE_LEAF) {
       const int tiles = L.kind == LK_OZAKI ? L.oz.total_tiles : L.gm.total_tiles;
       const int prods = L.kind == LK_OZAKI ? L.oz.ndesc : L.gm.ndesc;

@@ -74,7 +74,7 @@ Launch stage2_launch(LaunchKind kind, const FmAlgDesc* alg, std::vector<FmOp> op
   Launch L;
   L.kind = kind;
   L.alg = alg;
-  L.stage = kind == LK_S2 ? FM_STAGE_S : FM_STAGE_T;
+  L.stage = kind == LK_S2 ? FM_STAGE_S : kind == LK_T2 ? FM_STAGE_T : FM_STAGE_W;
   L.st.rows = (int32_t)rows;
   L.st.cols = (int32_t)cols;
   L.st.rows2 = (int32_t)rows2;
@@ -510,15 +510,19 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
   // that of level l + 1.  Pairs are taken from the deepest level up (deeper
   // levels move more data); a pair needs level l + 1 free of peeling, which
   // would read the level-(l+1) operands that a two-level stage never stores.
-  std::vector<char> s2(L, 0), t2(L, 0);
+  // w2[l]: the W stages of levels l + 1 and l run as one launch, and the
+  // level-(l+1) products are never stored.
+  std::vector<char> s2(L, 0), t2(L, 0), w2(L, 0);
   if (p.fuse) {
     const bool fs = (dt == FM_F64 ? alg->s2_d : alg->s2_f) != nullptr;
     const bool ft = (dt == FM_F64 ? alg->t2_d : alg->t2_f) != nullptr;
+    const bool fw = (dt == FM_F64 ? alg->w2_d : alg->w2_f) != nullptr;
     for (int l = L - 2; l >= 0;) {
       const Dims c = dims[l + 1];
-      if (c.m % alg->m == 0 && c.k % alg->k == 0 && c.n % alg->n == 0 && (fs || ft)) {
+      if (c.m % alg->m == 0 && c.k % alg->k == 0 && c.n % alg->n == 0 && (fs || ft || fw)) {
         s2[l] = fs;
         t2[l] = ft;
+        w2[l] = fw;
         l -= 2;
       } else {
         l -= 1;
@@ -544,7 +548,7 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
     const int64_t tv_off = ws;
     if (t_second) ws += Q * alg->b_nviews * t_slab;
     m_off[l] = ws;
-    ws += Q * R * m_slab[l];
+    if (!w2[l]) ws += Q * R * m_slab[l];
     lv[l + 1].reserve(Q * R);
     for (int64_t q = 0; q < Q; ++q) {
       Node& nd = lv[l][q];
@@ -565,7 +569,7 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
           ch.b = op_make(FM_ROLE_WS, nd.b_views + alg->b_view_slot[bi] * t_slab, t_ld);
         else
           ch.b = op_sub(nd.b, (bi / alg->n) * c.k, (bi % alg->n) * c.n);
-        ch.c = op_make(FM_ROLE_WS, m_off[l] + (q * R + r) * m_slab[l], m_ld[l]);
+        ch.c = w2[l] ? virt : op_make(FM_ROLE_WS, m_off[l] + (q * R + r) * m_slab[l], m_ld[l]);
         double mu = round_scalar(nd.mu * round_scalar(alg->a_coef(r, p.x), dt), dt);
         ch.mu = round_scalar(mu * round_scalar(alg->b_coef(r, p.x), dt), dt);
         lv[l + 1].push_back(ch);
@@ -606,8 +610,19 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
     const double bl = l == 0 ? round_scalar(p.beta, dt) : 0.0;
     std::vector<FmOp> ops;
     for (const Node& nd : lv[l]) ops.push_back(nd.c);
-    out.w[l] = stage_launch(dt, LK_W, alg, std::move(ops), c.m, c.n, m_off[l], m_ld[l], m_slab[l],
-                            p.x, bl);
+    if (w2[l]) {
+      // levels l + 1 and l in one launch; the level-(l+1) launch is dropped
+      const Dims c2 = dims[l + 2];
+      out.w[l + 1] = Launch();
+      out.w[l + 1].kind = LK_NONE;
+      Launch W = stage2_launch(LK_W2, alg, std::move(ops), c.m, c.n, c2.m, c2.n, m_off[l + 1],
+                               m_ld[l + 1], m_slab[l + 1], 0, p.x);
+      W.st.beta = bl;
+      out.w[l] = std::move(W);
+    } else {
+      out.w[l] = stage_launch(dt, LK_W, alg, std::move(ops), c.m, c.n, m_off[l], m_ld[l],
+                              m_slab[l], p.x, bl);
+    }
     for (const Node& nd : lv[l]) peel_descs(nd, dims[l], alg->m, alg->k, alg->n, bl, out.peels[l]);
   }
   out.depth = L;
@@ -644,7 +659,7 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
     std::vector<FmGemmDesc> peel;
     for (SubPlan& s : subs) {
       if (s.depth <= l) continue;
-      P.launches.push_back(std::move(s.w[l]));
+      if (s.w[l].kind != LK_NONE) P.launches.push_back(std::move(s.w[l]));
       peel.insert(peel.end(), s.peels[l].begin(), s.peels[l].end());
     }
     // peeling products stay on the plain kernel in OZAKI mode (FP64) and
@@ -780,6 +795,7 @@ fm_status Plan::run(const void* A, const void* B, void* C, void* ws, hipStream_t
         case LK_T: fn = d ? g->t_d : g->t_f; break;
         case LK_S2: fn = d ? g->s2_d : g->s2_f; break;
         case LK_T2: fn = d ? g->t2_d : g->t2_f; break;
+        case LK_W2: fn = d ? g->w2_d : g->w2_f; break;
         default: fn = d ? g->w_d : g->w_f; break;
       }
       e = fn(a, s);

@@ -19,7 +19,7 @@ fm_status hip_err(hipError_t e, const char* what);
 const std::string& last_error();
 void clear_error();
 
-enum LaunchKind { LK_S, LK_T, LK_W, LK_GEMM, LK_OZAKI, LK_S2, LK_T2 };
+enum LaunchKind { LK_S, LK_T, LK_W, LK_GEMM, LK_OZAKI, LK_S2, LK_T2, LK_W2, LK_NONE };
 
 // One kernel launch.  Descriptors stay on the host until Plan::upload packs
 // them into the plan's device buffer.

@@ -251,6 +251,78 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2(FmStageArgs a) 
 // (NB * NB values per lane in registers).
 constexpr int FM_STAGE2_MAX_BLOCKS = 9;
 
+// Two recursion levels of W in one pass (levels l+1 then l), for nodes q
+// of level l: the level-(l+1) products M1[q,r] (r < R, blocks rows x cols)
+// are never stored.  A work group takes P consecutive rows of one column of
+// the level-(l+2) block position and one node:
+//   1. for every r, the level-(l+1) W of node q*R + r at those positions:
+//      w_combine over M2[(q*R + r)*R + r2] (r2 < R) gives the m*n sub-blocks
+//      t of M1[q,r], kept in LDS;
+//   2. for every sub-block t, the level-l W: w_combine over M1[q,r][t]
+//      (r < R) gives C_q's blocks ij at sub-block t (+ beta * C).
+// Same operations in the same order as the two one-level W stages.
+template <class Cmb, typename T>
+constexpr int fm_w2_rows() {
+  // rows per work group: LDS holds R * m*n * P values, at most 64 KB
+  constexpr int per = Cmb::R * Cmb::m * Cmb::n * (int)sizeof(T);
+  return per * 32 <= 65536 ? 32 : per * 16 <= 65536 ? 16 : per * 8 <= 65536 ? 8 : 0;
+}
+
+template <class Cmb, typename T, bool BETA>
+__global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage2(FmStageArgs a) {
+  constexpr int R = Cmb::R, MN = Cmb::m * Cmb::n, P = fm_w2_rows<Cmb, T>();
+  static_assert(P > 0, "level-(l+1) products do not fit in LDS");
+  __shared__ T m1[R][MN][P];
+  const int q = blockIdx.y;
+  const int rows2 = a.rows2;
+  const int chunks = (rows2 + P - 1) / P;
+  const int j = blockIdx.x / chunks;           // column within the sub-block
+  const int i0 = (blockIdx.x - j * chunks) * P;  // first row
+  const T* mb = reinterpret_cast<const T*>(a.bases.p[FM_ROLE_WS]) + a.out_off +
+                (int64_t)q * R * R * a.out_slab;
+  const int64_t col_off = (int64_t)j * a.out_ld;
+  // 1. level-(l+1) W, one (row, r) item per thread and pass
+  for (int item = threadIdx.x; item < P * R; item += FM_STAGE_THREADS) {
+    const int p = item % P, r = item / P;
+    const int i = i0 + p;
+    if (i >= rows2) continue;
+    T mv[R];
+    const T* src = mb + (int64_t)r * R * a.out_slab + i + col_off;
+#pragma unroll
+    for (int r2 = 0; r2 < R; ++r2) mv[r2] = src[(int64_t)r2 * a.out_slab];
+    T out[MN];
+    Cmb::w_combine(mv, out, a.x);
+#pragma unroll
+    for (int t = 0; t < MN; ++t) m1[r][t][p] = out[t];
+  }
+  __syncthreads();
+  // 2. level-l W, one (row, sub-block t) item per thread and pass
+  const FmOp op = a.ops[q];
+  T* c = reinterpret_cast<T*>(fm_base(a.bases, op.role)) + op.off;
+  const int64_t ld = op.ld;
+  const T beta = (T)a.beta;
+  for (int item = threadIdx.x; item < P * MN; item += FM_STAGE_THREADS) {
+    const int p = item % P, t = item / P;
+    const int i = i0 + p;
+    if (i >= rows2) continue;
+    T mv[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) mv[r] = m1[r][t][p];
+    T out[MN];
+    Cmb::w_combine(mv, out, a.x);
+    const int ti = t / Cmb::n, tj = t % Cmb::n;
+    const int64_t r0 = (int64_t)ti * rows2 + i, c0 = (int64_t)tj * a.cols2 + j;
+#pragma unroll
+    for (int b = 0; b < MN; ++b) {
+      const int bi = b / Cmb::n, bj = b % Cmb::n;
+      T* dst = c + (int64_t)bi * a.rows + r0 + ((int64_t)bj * a.cols + c0) * ld;
+      T v = out[b];
+      if constexpr (BETA) v = fm_add(v, fm_mul(beta, *dst));
+      *dst = v;
+    }
+  }
+}
+
 inline dim3 fm_stage_grid(const FmStageArgs& a, int vec) {
   const uint64_t total = (uint64_t)(a.rows / vec) * (uint64_t)a.cols;
   uint64_t gx = (total + FM_STAGE_THREADS - 1) / FM_STAGE_THREADS;
@@ -314,6 +386,26 @@ hipError_t fm_launch_in_stage2(const FmStageArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <class Cmb, typename T>
+hipError_t fm_launch_w_stage2(const FmStageArgs& a, hipStream_t s) {
+  constexpr int P = fm_w2_rows<Cmb, T>();
+  const unsigned chunks = (unsigned)((a.rows2 + P - 1) / P);
+  const dim3 grid(chunks * (unsigned)a.cols2, (unsigned)a.nodes, 1);
+  if (a.beta != 0.0)
+    hipLaunchKernelGGL((fm_w_stage2<Cmb, T, true>), grid, dim3(FM_STAGE_THREADS), 0, s, a);
+  else
+    hipLaunchKernelGGL((fm_w_stage2<Cmb, T, false>), grid, dim3(FM_STAGE_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class Cmb, typename T>
+constexpr FmStageLaunch fm_w2_fn() {
+  if constexpr (fm_w2_rows<Cmb, T>() > 0)
+    return &fm_launch_w_stage2<Cmb, T>;
+  else
+    return nullptr;
+}
+
 template <class Cmb, int SIDE>
 constexpr bool fm_stage2_ok() {
   return (SIDE == 0 ? Cmb::m * Cmb::k : Cmb::k * Cmb::n) <= FM_STAGE2_MAX_BLOCKS &&
@@ -356,6 +448,7 @@ constexpr FmStageLaunch fm_stage2_fn() {
         fm_stage2_fn<GNS::Combine, double, 0>(), fm_stage2_fn<GNS::Combine, double, 1>(),     \
         fm_stage2_fn<GNS::Combine, float, 0>(), fm_stage2_fn<GNS::Combine, float, 1>(),       \
         GNS::Combine::a_nviews, GNS::Combine::b_nviews, GNS::Combine::a_view_slot,            \
-        GNS::Combine::b_view_slot};                                                          \
+        GNS::Combine::b_view_slot, fm_w2_fn<GNS::Combine, double>(),                          \
+        fm_w2_fn<GNS::Combine, float>()};                                                    \
     return &d;                                                                               \
   }

@@ -71,7 +71,9 @@ def test_workspace_query_and_depth_limit():
     m1 = 2048 * 2048
     one_level = (22 * 2048 * 4096 + 20 * 4096 * 2048 + 26 * m1          # level 1
                  + 26 * (22 * s2 + 20 * t2) + 676 * m2)                 # level 2
-    two_level = 26 * m1 + 26 * (22 * s2 + 20 * t2) + 26 * (4 * s2 + 6 * t2) + 676 * m2
+    # two-level stages store neither the level-1 S/T operands nor the level-1
+    # products (the two W stages run as one)
+    two_level = 26 * (22 * s2 + 20 * t2) + 26 * (4 * s2 + 6 * t2) + 676 * m2
     try:
         fastmm.set_stage_fusion(False)
         assert fastmm.workspace_bytes("grey424_26_206", 8192, 8192, 8192, 2) == 8 * one_level

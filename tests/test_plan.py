@@ -25,8 +25,8 @@ def test_bench_workload_plan():
     # (FmTileCfg 21, two work groups per CU) on 256 CUs: 21 full rounds
     # (672 products) + 4 products on 64 x 64 tiles (FmTileCfg 1)
     p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
-    assert p["launches"] == {"S": 1, "T": 1, "W": 2, "LEAF": 2, "PEEL": 0}
-    assert p["two_level_stages"] == 2
+    assert p["launches"] == {"S": 1, "T": 1, "W": 1, "LEAF": 2, "PEEL": 0}
+    assert p["two_level_stages"] == 3
     assert p["leaf"] == [{"cfg": 21, "products": 672, "tiles": 10752},
                          {"cfg": 1, "products": 4, "tiles": 256}]
     fastmm.set_schedule(0, 0, False)
@@ -37,7 +37,7 @@ def test_bench_workload_plan():
 def test_one_level_stages_when_middle_level_peels():
     p = fastmm.plan_info("strassen", 130, 130, 130, 2)
     assert p["two_level_stages"] == 0
-    assert p["launches"]["S"] == 2 and p["launches"]["T"] == 2
+    assert p["launches"]["S"] == 2 and p["launches"]["T"] == 2 and p["launches"]["W"] == 2
     assert p["launches"]["PEEL"] == 1   # 130 splits evenly, 65 does not: level 1 peels
     fastmm.set_stage_fusion(False)
     p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
@@ -96,3 +96,6 @@ def test_stage_bytes_match_closed_forms():
     b = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)["stage_bytes"]
     assert b["S"] == 8 * 512 * 2048 * (64 + 26 * 22 + 22 * 4)
     assert b["T"] == 8 * 2048 * 512 * (64 + 26 * 20 + 20 * 6)
+    # two-level W: read the 676 level-2 products once, write C's 16 x 16
+    # sub-blocks once
+    assert b["W"] == 8 * 512 * 512 * (676 + 256)
