@@ -99,3 +99,24 @@ def test_stage_bytes_match_closed_forms():
     # two-level W: read the 676 level-2 products once, write C's 16 x 16
     # sub-blocks once
     assert b["W"] == 8 * 512 * 512 * (676 + 256)
+
+
+def test_benchmark_output_format_reads_reference_data():
+    """fastmm.benchmark's parser reads the reference's own data files
+    (data/README.md:15-27) and its eff_gflops matches the README's formula on
+    a published point: Strassen L=1 at 2000^3 took 667.291 ms = 23.97 GFLOP/s."""
+    import os
+    from fastmm import benchmark as bm
+    path = os.path.join(os.path.dirname(__file__), "golden", "matlab_square_sample.txt")
+    data = bm.parse(open(path).read())
+    assert set(data) == {"MKL_0", "STRASSEN_1", "STRASSEN_2"}
+    pt = [p for p in data["STRASSEN_1"] if p[0] == 2000][0]
+    assert pt[:4] == (2000, 2000, 2000, 1) and pt[4] == 667.291
+    assert abs(bm.eff_gflops(pt) - 23.97) < 0.01
+    # round trip of our own output line
+    line = "%s_%d = [ 8192 8192 8192 2 14.2 ;  4096 4096 4096 2 2.1 ; ];" % (
+        bm.enum_name("grey424_26_206"), 2)
+    assert line.startswith("FAST424_26_206_2 = [")
+    assert bm.parse(line) == {"FAST424_26_206_2": [(8192, 8192, 8192, 2, 14.2),
+                                                    (4096, 4096, 4096, 2, 2.1)]}
+    assert bm.enum_name("strassen") == "STRASSEN" and bm.enum_name("MKL") == "MKL"
