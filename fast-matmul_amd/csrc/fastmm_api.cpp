@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "plan.hpp"
+#include "scaling.hpp"
 
 using namespace fmplan;
 
@@ -164,6 +165,33 @@ fm_status arena_get(size_t bytes, void** p) {
   return FM_OK;
 }
 
+// Run `fn` on stream s; with ms_out, bracket it with events and wait.
+template <class F>
+fm_status timed(hipStream_t s, float* ms_out, F&& fn) {
+  if (!ms_out) return fn();
+  hipEvent_t e0, e1;
+  hipError_t e = hipEventCreate(&e0);
+  if (e != hipSuccess) return hip_err(e, "hipEventCreate");
+  e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return hip_err(e, "hipEventCreate");
+  }
+  (void)hipEventRecord(e0, s);
+  fm_status st = fn();
+  (void)hipEventRecord(e1, s);
+  if (st == FM_OK) {
+    e = hipEventSynchronize(e1);
+    if (e != hipSuccess)
+      st = hip_err(e, "hipEventSynchronize");
+    else
+      (void)hipEventElapsedTime(ms_out, e0, e1);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return st;
+}
+
 fm_status timed_run(const Plan& P, const void* A, const void* B, void* C, void* ws, hipStream_t s,
                     float* ms_out) {
   if (!ms_out) return P.run(A, B, C, ws, s);
@@ -275,6 +303,98 @@ fm_status fm_fastmatmul(const char* alg_name, fm_dtype dtype, fm_mode mode, int 
   st = plan_workspace(*P, workspace, workspace_bytes, &ws);
   if (st != FM_OK) return st;
   return timed_run(*P, A, B, C, ws, (hipStream_t)stream, ms_out);
+}
+
+fm_status fm_fastmatmul_scaled(const char* alg_name, fm_dtype dtype, fm_mode mode, int m, int k,
+                               int n, const void* A, int64_t lda, const void* B, int64_t ldb,
+                               void* C, int64_t ldc, int num_steps, double x, double alpha,
+                               double beta, int scaling_type, int scaling_steps, void* stream,
+                               void* workspace, size_t workspace_bytes, float* ms_out) {
+  clear_error();
+  if (ms_out) *ms_out = 0.f;
+  const FmAlgDesc* alg = fm_find_alg(alg_name);
+  if (!alg)
+    return set_err(FM_ERR_UNKNOWN_ALG, std::string("unknown algorithm: ") + (alg_name ? alg_name : "(null)"));
+  fm_status st = check_args(dtype, m, k, n, A, lda, B, ldb, C, ldc);
+  if (st != FM_OK) return st;
+  if (num_steps < 0) return set_err(FM_ERR_INVALID_ARG, "negative num_steps");
+  if (scaling_type < 0 || scaling_type > 3 || scaling_steps < 0)
+    return set_err(FM_ERR_INVALID_ARG, "scaling_type must be 0..3 and scaling_steps >= 0");
+  st = check_mode(dtype, mode);
+  if (st != FM_OK) return st;
+  if (m == 0 || n == 0) return FM_OK;
+  if (k == 0) return set_err(FM_ERR_SHAPE, "k == 0");
+  std::lock_guard<std::mutex> lock(fm_api_mutex());
+  // the multiply of the scaled copies: contiguous operands, product into C
+  // (beta = 0) or into a workspace temporary
+  const bool temp = beta != 0.0;
+  Plan* P = nullptr;
+  st = get_plan(alg_name, alg, dtype, mode, m, k, n, m, k, temp ? m : ldc, num_steps, x, 1.0, 0.0,
+                &P);
+  if (st != FM_OK) return st;
+  const size_t es = (size_t)elem_size(dtype);
+  auto al = [](size_t v) { return (v + 255) / 256 * 256; };
+  const size_t L = (size_t)std::max(m, std::max(k, n));
+  size_t off = al(P->ws_bytes);
+  const size_t o_a = off;
+  off += al(es * (size_t)m * k);
+  const size_t o_b = off;
+  off += al(es * (size_t)k * n);
+  const size_t o_t = off;
+  if (temp) off += al(es * (size_t)m * n);
+  const size_t o_r = off;
+  off += al(es * m);
+  const size_t o_s = off;
+  off += al(es * n);
+  const size_t o_f = off;
+  off += al(es * L);
+  const size_t o_f2 = off;
+  off += al(es * L);
+  const size_t o_u1 = off;
+  off += al(8 * L);
+  const size_t o_u2 = off;
+  off += al(8 * L);
+  void* ws = workspace;
+  if (workspace) {
+    if (workspace_bytes < off)
+      return set_err(FM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(off) + " bytes");
+  } else {
+    st = arena_get(off, &ws);
+    if (st != FM_OK) return st;
+  }
+  char* w = (char*)ws;
+  hipStream_t s = (hipStream_t)stream;
+  const int f64 = dtype == FM_F64;
+  return timed(s, ms_out, [&]() -> fm_status {
+    hipError_t e = fm_scaling_copy(f64, A, lda, w + o_a, m, m, k, s);
+    if (e == hipSuccess) e = fm_scaling_copy(f64, B, ldb, w + o_b, k, k, n, s);
+    if (e != hipSuccess) return hip_err(e, "scaling copy");
+    FmScaleArgs sa;
+    sa.A = w + o_a;
+    sa.B = w + o_b;
+    sa.lda = m;
+    sa.ldb = k;
+    sa.m = m;
+    sa.k = k;
+    sa.n = n;
+    sa.type = scaling_type;
+    sa.steps = scaling_steps;
+    sa.r = w + o_r;
+    sa.s = w + o_s;
+    sa.f = w + o_f;
+    sa.f2 = w + o_f2;
+    sa.u1 = w + o_u1;
+    sa.u2 = w + o_u2;
+    e = fm_scaling_run(f64, sa, s);
+    if (e != hipSuccess) return hip_err(e, "scaling");
+    void* prod = temp ? (void*)(w + o_t) : C;
+    const int64_t ldp = temp ? m : ldc;
+    fm_status r = P->run(w + o_a, w + o_b, prod, w, s);
+    if (r != FM_OK) return r;
+    e = fm_scaling_post(f64, prod, ldp, C, ldc, m, n, w + o_r, w + o_s, alpha, beta, s);
+    if (e != hipSuccess) return hip_err(e, "scaling post-process");
+    return FM_OK;
+  });
 }
 
 fm_status fm_workspace_query(const char* alg_name, fm_dtype dtype, fm_mode mode, int m, int k,

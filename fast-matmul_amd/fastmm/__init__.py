@@ -25,7 +25,7 @@ from . import _lib
 from . import tables
 from ._lib import LIB, FastMMError, check  # noqa: F401
 
-__all__ = ["Matrix", "fastmatmul", "gemm", "MatMul", "RunAlgorithm", "alg", "algorithm_names",
+__all__ = ["Matrix", "fastmatmul", "fastmatmul_scaled", "gemm", "MatMul", "RunAlgorithm", "alg", "algorithm_names",
            "workspace_bytes", "release_caches"]
 
 _DT = {torch.float64: _lib.FM_F64, torch.float32: _lib.FM_F32}
@@ -111,6 +111,42 @@ def fastmatmul(alg_name: str, A: torch.Tensor, B: torch.Tensor, C: Optional[torc
                            float(x), float(alpha), float(beta), _stream_handle(A), ws_ptr,
                            ws_bytes, ctypes.byref(ms) if timed else None)
     check(st)
+    return (C, ms.value) if timed else C
+
+
+def fastmatmul_scaled(alg_name: str, A: torch.Tensor, B: torch.Tensor,
+                      C: Optional[torch.Tensor] = None, num_steps: int = 1, x: float = 1e-8,
+                      alpha: float = 1.0, beta: float = 0.0, mode: str = "plain",
+                      scaling: str = "outside_inside", scaling_steps: int = 1,
+                      timed: bool = False):
+    """fastmatmul on diagonally scaled copies of A and B, scaled back
+    (scaling/scaling.hpp:105-196): C := alpha diag(r) (A' B') diag(s) + beta C.
+    ``scaling`` is one of outside_inside, inside_outside, inside, outside."""
+    A = _col_major(A, "A")
+    B = _col_major(B, "B")
+    m, k = A.shape
+    k2, n = B.shape
+    if k2 != k:
+        raise ValueError("inner dimensions differ: %d vs %d" % (k, k2))
+    if A.dtype != B.dtype or A.dtype not in _DT:
+        raise TypeError("A and B must share a float64/float32 dtype")
+    if C is None:
+        C = empty(m, n, A.dtype, A.device)
+        if beta != 0.0:
+            raise ValueError("beta != 0 needs an input C")
+    C = _col_major(C, "C")
+    if C.shape != (m, n) or C.dtype != A.dtype:
+        raise ValueError("C must be %d x %d of dtype %s" % (m, n, A.dtype))
+    if scaling not in _lib.SCALINGS:
+        raise ValueError("scaling must be one of %s" % sorted(_lib.SCALINGS))
+    ms = ctypes.c_float(0.0)
+    check(LIB.fm_fastmatmul_scaled(alg_name.encode(), _DT[A.dtype], _lib.MODES[mode], m, k, n,
+                                   ctypes.c_void_p(A.data_ptr()), _ld(A),
+                                   ctypes.c_void_p(B.data_ptr()), _ld(B),
+                                   ctypes.c_void_p(C.data_ptr()), _ld(C), int(num_steps), float(x),
+                                   float(alpha), float(beta), _lib.SCALINGS[scaling],
+                                   int(scaling_steps), _stream_handle(A), None, 0,
+                                   ctypes.byref(ms) if timed else None))
     return (C, ms.value) if timed else C
 
 

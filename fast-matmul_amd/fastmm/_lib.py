@@ -31,8 +31,11 @@ EXPORTS = (
     "fm_release_caches", "fm_profile_enable", "fm_profile_read", "fm_set_gemm_tile",
     "fm_set_ozaki_slices", "fm_batch_create", "fm_batch_run", "fm_batch_choice",
     "fm_batch_workspace_bytes", "fm_batch_destroy", "fm_autotune_estimate",
-    "fm_set_stage_fusion", "fm_set_schedule", "fm_plan_query",
+    "fm_set_stage_fusion", "fm_set_schedule", "fm_plan_query", "fm_fastmatmul_scaled",
 )
+
+# scaling rules of scaling/scaling.hpp:17-22
+SCALINGS = {"outside_inside": 0, "inside_outside": 1, "inside": 2, "outside": 3}
 
 STAGES = ("S", "T", "W", "LEAF", "PEEL")
 
@@ -74,6 +77,11 @@ def _load():
                                   vp, i64, vp, i64, vp, i64, c_int, c_double, c_double,
                                   c_double, vp, vp, ctypes.c_size_t,
                                   ctypes.POINTER(ctypes.c_float)]
+    lib.fm_fastmatmul_scaled.restype = c_int
+    lib.fm_fastmatmul_scaled.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int,
+                                         vp, i64, vp, i64, vp, i64, c_int, c_double, c_double,
+                                         c_double, c_int, c_int, vp, vp, ctypes.c_size_t,
+                                         ctypes.POINTER(ctypes.c_float)]
     lib.fm_workspace_query.restype = c_int
     lib.fm_workspace_query.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int,
                                        c_int, ctypes.POINTER(ctypes.c_size_t)]

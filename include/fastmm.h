@@ -164,6 +164,23 @@ fm_status fm_set_ozaki_slices(int slices);
  * automatic choice.  Cached plans are dropped. */
 fm_status fm_set_gemm_tile(int cfg);
 
+/* Diagonal scaling around the fast multiply (scaling/scaling.hpp:105-196,
+ * scaling/fast_scaling.cpp): copies of A and B are scaled, scaling_steps
+ * times, by the reference's OUTSIDE_INSIDE (0), INSIDE_OUTSIDE (1), INSIDE (2)
+ * or OUTSIDE (3) rule -- outer: rows of A and columns of B divided by their
+ * largest magnitude; inner: column k of A divided and row k of B multiplied by
+ * sqrt(max|A[:,k]| / max|B[k,:]|) -- then multiplied with `alg`, and the
+ * product's rows and columns multiplied back (PostProcessScaling):
+ * C := alpha * diag(r) (A' B') diag(s) + beta * C.  Improves accuracy when
+ * rows of A or columns of B differ widely in magnitude.  A row or column of
+ * zeros keeps factor 1.  Workspace: the plan's plus copies of A and B (and
+ * of the product when beta != 0); workspace == NULL uses the arena. */
+fm_status fm_fastmatmul_scaled(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k, int n,
+                               const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                               int64_t ldc, int num_steps, double x, double alpha, double beta,
+                               int scaling_type, int scaling_steps, void* stream, void* workspace,
+                               size_t workspace_bytes, float* ms_out);
+
 /* Shape of the plan fm_fastmatmul would run (built on the host, nothing
  * launched; contiguous operands): launches per fm_stage_kind, algorithmic
  * HBM bytes per fm_stage_kind (every source element read once, every output
