@@ -57,12 +57,14 @@ __device__ __forceinline__ int fm_xcd_remap(int wg, int nwg) {
 // barrier wait is covered by the other's MFMAs).  UNR: unroll of the
 // 4-deep K steps inside one LDS slice (less unrolling, fewer live operand
 // registers).
-// PIPE: software-pipelined main loop (see the loop below).  FULL: every
+// PIPE: software-pipelined main loop (see the loop below); PIPE == 2 also
+// interleaves the next step's LDS reads with the current step's MFMAs
+// (one read, then two MFMAs) through scheduling-group barriers.  FULL: every
 // product is a whole number of tiles and K slices with 16-byte aligned
 // columns (the launch checks), so only the unpredicated 16-byte load path
 // is compiled.
 template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW = 1, int UNR = BK / 4,
-          bool PIPE = false, bool FULL = false>
+          int PIPE = 0, bool FULL = false>
 __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs args) {
   using Op = FmMfma<T>;
   using acc_t = typename Op::acc_t;
@@ -243,6 +245,14 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
       for (int k4 = 0; k4 < NS4 - 1; ++k4) {
         frag(cur, k4 + 1, (k4 + 1) & 1);
         mma_step(k4 & 1);
+        if constexpr (PIPE == 2) {
+#pragma unroll
+          for (int r = 0; r < TM + TN; ++r) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+            __builtin_amdgcn_sched_group_barrier(0x008, (TM * TN) / (TM + TN), 0);  // MFMAs
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
       // the last step's fragments are in registers: publish the next slice
       if (kt + 1 < nk) {
@@ -318,7 +328,7 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
 }
 
 template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW = 1, int UNR = BK / 4,
-          bool PIPE = false>
+          int PIPE = 0>
 static void fm_gemm_go(const FmGemmArgs& a, hipStream_t s) {
   const int grid = a.grid > 0 && a.grid < a.total_tiles ? a.grid : a.total_tiles;
   constexpr int NT = 64 * WM * WN, VL = 16 / sizeof(T);
@@ -358,10 +368,12 @@ static hipError_t fm_gemm_launch(int cfg, const FmGemmArgs& a, hipStream_t s) {
     case FM_TILE_256x128_W8_U1: fm_gemm_go<T, 256, 128, 16, 4, 2, 1, 1>(a, s); break;
     case FM_TILE_128x128_O2_U1: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 1>(a, s); break;
     case FM_TILE_128x256_W8_U1: fm_gemm_go<T, 128, 256, 16, 2, 4, 1, 1>(a, s); break;
-    case FM_TILE_PP_256x128_W8: fm_gemm_go<T, 256, 128, 16, 4, 2, 1, 4, true>(a, s); break;
-    case FM_TILE_PP_128x256_W8: fm_gemm_go<T, 128, 256, 16, 2, 4, 1, 4, true>(a, s); break;
-    case FM_TILE_PP_128x128_W8: fm_gemm_go<T, 128, 128, 16, 2, 4, 1, 4, true>(a, s); break;
-    case FM_TILE_PP_128x128_O2: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 4, true>(a, s); break;
+    case FM_TILE_PP_256x128_W8: fm_gemm_go<T, 256, 128, 16, 4, 2, 1, 4, 1>(a, s); break;
+    case FM_TILE_PP_128x256_W8: fm_gemm_go<T, 128, 256, 16, 2, 4, 1, 4, 1>(a, s); break;
+    case FM_TILE_PP_128x128_W8: fm_gemm_go<T, 128, 128, 16, 2, 4, 1, 4, 1>(a, s); break;
+    case FM_TILE_PP_128x128_O2: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 4, 1>(a, s); break;
+    case FM_TILE_PI_128x128_O2: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 4, 2>(a, s); break;
+    case FM_TILE_PI_128x256_W8: fm_gemm_go<T, 128, 256, 16, 2, 4, 1, 4, 2>(a, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

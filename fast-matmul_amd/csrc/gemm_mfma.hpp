@@ -55,7 +55,10 @@ enum FmTileCfg : int {
   FM_TILE_PP_128x256_W8 = 19,  // BK 16, 2x4 waves
   FM_TILE_PP_128x128_W8 = 20,  // BK 16, 2x4 waves
   FM_TILE_PP_128x128_O2 = 21,  // BK 16, 2x2 waves, registers for 2 waves per SIMD
-  FM_TILE_NUM = 22
+  // as PP, with LDS reads interleaved with MFMAs by scheduling-group barriers
+  FM_TILE_PI_128x128_O2 = 22,  // BK 16, 2x2 waves, registers for 2 waves per SIMD
+  FM_TILE_PI_128x256_W8 = 23,  // BK 16, 2x4 waves
+  FM_TILE_NUM = 24
 };
 
 // K slice depth of each configuration
@@ -72,7 +75,7 @@ inline void fm_tile_dims(int cfg, int* bm, int* bn) {
                                         {128, 256}, {256, 256}, {64, 64},  {128, 128},
                                         {128, 64},  {64, 128},  {128, 128}, {256, 128},
                                         {128, 128}, {128, 256}, {256, 128}, {128, 256},
-                                        {128, 128}, {128, 128}};
+                                        {128, 128}, {128, 128}, {128, 128}, {128, 256}};
   *bm = d[cfg][0];
   *bn = d[cfg][1];
 }

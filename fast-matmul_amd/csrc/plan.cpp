@@ -260,17 +260,17 @@ struct TileCand {
 // what the register allocation (waves per SIMD x 4 / waves per group) and
 // the 160 KB of LDS allow.
 const TileCand kF64Tiles[] = {
-    {FM_TILE_PP_128x128_O2, FM_TILE_128x128_O2_U1, 1.00, 2},
-    {FM_TILE_128x256_W8, FM_TILE_128x256_W8_U1, 0.99, 1},
-    {FM_TILE_256x128_W8, FM_TILE_256x128_W8_U1, 0.975, 1},
-    {FM_TILE_128x128_W16, FM_TILE_128x128_W16, 0.92, 1},
-    {FM_TILE_128x64_W8, FM_TILE_128x64_W8, 0.90, 2},
-    {FM_TILE_64x64, FM_TILE_64x64, 0.80, 3}};
+    {FM_TILE_PI_128x128_O2, FM_TILE_128x128_O2_U1, 1.00, 2},
+    {FM_TILE_128x256_W8, FM_TILE_128x256_W8_U1, 0.97, 1},
+    {FM_TILE_256x128_W8, FM_TILE_256x128_W8_U1, 0.96, 1},
+    {FM_TILE_128x128_W16, FM_TILE_128x128_W16, 0.90, 1},
+    {FM_TILE_128x64_W8, FM_TILE_128x64_W8, 0.88, 2},
+    {FM_TILE_64x64, FM_TILE_64x64, 0.78, 3}};
 const TileCand kF32Tiles[] = {
-    {FM_TILE_256x256_W16, -1, 1.00, 1},
-    {FM_TILE_PP_128x128_W8, FM_TILE_128x128_W8, 0.995, 1},
-    {FM_TILE_PP_128x256_W8, FM_TILE_PP_128x256_W8, 0.96, 1},
-    {FM_TILE_256x128_W8, FM_TILE_256x128_W8, 0.95, 1},
+    {FM_TILE_PI_128x128_O2, FM_TILE_128x128_W8, 1.00, 3},
+    {FM_TILE_256x256_W16, -1, 0.98, 1},
+    {FM_TILE_PP_128x256_W8, FM_TILE_PP_128x256_W8, 0.94, 1},
+    {FM_TILE_256x128_W8, FM_TILE_256x128_W8, 0.93, 1},
     {FM_TILE_128x128_W8, FM_TILE_128x128_W8, 0.79, 2},
     {FM_TILE_64x128, FM_TILE_64x128, 0.74, 3},
     {FM_TILE_128x64, FM_TILE_128x64, 0.71, 3},

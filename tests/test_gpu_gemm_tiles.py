@@ -12,7 +12,7 @@ from cases import make_inputs
 pytestmark = pytest.mark.gpu
 
 DEV = "cuda:0"
-N_CFG = 22  # FM_TILE_NUM
+N_CFG = 24  # FM_TILE_NUM
 
 
 def _dev(a, fm):
