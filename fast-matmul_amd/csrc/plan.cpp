@@ -267,8 +267,8 @@ const TileCand kF64Tiles[] = {
     {FM_TILE_128x64_W8, FM_TILE_128x64_W8, 0.88, 2},
     {FM_TILE_64x64, FM_TILE_64x64, 0.78, 3}};
 const TileCand kF32Tiles[] = {
-    {FM_TILE_PI_128x128_O2, FM_TILE_128x128_W8, 1.00, 3},
-    {FM_TILE_256x256_W16, -1, 0.98, 1},
+    {FM_TILE_256x256_W16, -1, 1.00, 1},
+    {FM_TILE_PI_128x128_O2, FM_TILE_128x128_W8, 0.97, 3},
     {FM_TILE_PP_128x256_W8, FM_TILE_PP_128x256_W8, 0.94, 1},
     {FM_TILE_256x128_W8, FM_TILE_256x128_W8, 0.93, 1},
     {FM_TILE_128x128_W8, FM_TILE_128x128_W8, 0.79, 2},
