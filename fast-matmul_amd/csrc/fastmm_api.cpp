@@ -254,6 +254,12 @@ double launch_bytes(const Launch& L, double es) {
         b += es * ((double)d.M * d.K + (double)d.K * d.N + (double)d.M * d.N * (d.beta != 0.0 ? 2 : 1));
       return b;
     }
+    case LK_SPLIT: {
+      double b = 0;
+      for (const FmSplitDesc& d : L.sdescs)
+        b += es * ((double)d.M * d.K + (double)d.K * d.N + (double)d.M * d.N * (d.beta != 0.0 ? 2 : 1));
+      return b;
+    }
     case LK_OZAKI: {
       double b = 0;
       for (const FmOzakiDesc& d : L.odescs)
@@ -436,8 +442,10 @@ fm_status fm_plan_query(const char* alg_name, fm_dtype dtype, fm_mode mode, int 
     info->stage_bytes[L.stage] += launch_bytes(L, es);
     if (L.kind == LK_S2 || L.kind == LK_T2 || L.kind == LK_W2) info->two_level_stages += 1;
     if (L.stage == FM_STAGE_LEAF) {
-      const int tiles = L.kind == LK_OZAKI ? L.oz.total_tiles : L.gm.total_tiles;
-      const int prods = L.kind == LK_OZAKI ? L.oz.ndesc : L.gm.ndesc;
+      const int tiles = L.kind == LK_OZAKI   ? L.oz.total_tiles
+                        : L.kind == LK_SPLIT ? L.sp.total_tiles
+                                             : L.gm.total_tiles;
+      const int prods = L.kind == LK_OZAKI ? L.oz.ndesc : L.kind == LK_SPLIT ? L.sp.ndesc : L.gm.ndesc;
       if (nleaf < 2) {
         info->leaf_cfg[nleaf] = L.kind == LK_OZAKI ? -1 : L.cfg;
         info->leaf_products[nleaf] = prods;

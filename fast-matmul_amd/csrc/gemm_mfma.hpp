@@ -58,12 +58,22 @@ enum FmTileCfg : int {
   // as PP, with LDS reads interleaved with MFMAs by scheduling-group barriers
   FM_TILE_PI_128x128_O2 = 22,  // BK 16, 2x2 waves, registers for 2 waves per SIMD
   FM_TILE_PI_128x256_W8 = 23,  // BK 16, 2x4 waves
-  FM_TILE_NUM = 24
+  // split-bf16 leaves on bf16 slice planes prepared once per operand
+  // (FP32 data only): BK 32
+  FM_TILE_SP_128x128 = 24,     // 2x2 waves, one LDS buffer (two work groups per CU)
+  FM_TILE_SP_256x128 = 25,     // 4x2 waves, LDS double-buffered
+  FM_TILE_SP_128x256 = 26,     // 2x4 waves, LDS double-buffered
+  FM_TILE_SP_128x128_DB = 27,  // 2x2 waves, LDS double-buffered
+  FM_TILE_NUM = 28
 };
+
+// configurations that run on prepared slice planes
+inline bool fm_tile_planes(int cfg) { return cfg >= FM_TILE_SP_128x128 && cfg < FM_TILE_NUM; }
 
 // K slice depth of each configuration
 inline int fm_tile_bk(int cfg) {
-  return (cfg == FM_TILE_128x128_K32 || cfg == FM_TILE_128x128_K32_W8 || cfg == FM_TILE_64x64_K32)
+  return (cfg == FM_TILE_128x128_K32 || cfg == FM_TILE_128x128_K32_W8 || cfg == FM_TILE_64x64_K32 ||
+          fm_tile_planes(cfg))
              ? 32
              : 16;
 }
@@ -75,7 +85,8 @@ inline void fm_tile_dims(int cfg, int* bm, int* bn) {
                                         {128, 256}, {256, 256}, {64, 64},  {128, 128},
                                         {128, 64},  {64, 128},  {128, 128}, {256, 128},
                                         {128, 128}, {128, 256}, {256, 128}, {128, 256},
-                                        {128, 128}, {128, 128}, {128, 128}, {128, 256}};
+                                        {128, 128}, {128, 128}, {128, 128}, {128, 256},
+                                        {128, 128}, {256, 128}, {128, 256}, {128, 128}};
   *bm = d[cfg][0];
   *bn = d[cfg][1];
 }
@@ -84,3 +95,42 @@ hipError_t fm_gemm_launch_f64(int cfg, const FmGemmArgs& a, hipStream_t s);
 hipError_t fm_gemm_launch_f32(int cfg, const FmGemmArgs& a, hipStream_t s);
 // FP32 data, split-bf16 slice products (csrc/gemm_split.hip)
 hipError_t fm_gemm_launch_bf16x3(int cfg, const FmGemmArgs& a, hipStream_t s);
+
+// ---- split-bf16 products on prepared slice planes ----------------------------
+// An FP32 operand of `rows` x K (A: rows = M, row r = A[r, :]; B: rows = N,
+// row r = B[:, r]) is cut once into three bf16 slice planes in the
+// workspace.  Element (r, k), slice p (0 = leading) is the bf16 at byte
+//   dst + (r * Kp / 32 + k / 32) * 192 + p * 64 + (k % 32) * 2,
+// so each 32-deep K slice of a row is one 192-byte chunk holding the three
+// planes; rows [rows, rows_pad) and k in [K, Kp) hold zeros.
+struct FmSplitPrep {
+  FmOp src;             // FP32 source, column-major
+  int32_t rows, K;
+  int32_t rows_pad, Kp;  // rows rounded up to the tile, K rounded up to 32
+  int32_t kcontig;       // 1: K contiguous in the source (B), 0: rows contiguous (A)
+  int32_t pad;
+  int64_t dst;           // byte offset in the workspace
+};
+
+// One product C = alpha * A B + beta * C from the planes of its operands.
+struct FmSplitDesc {
+  FmOp c;
+  int32_t M, N, K, Kp;
+  int32_t tiles_m, tiles_n, tile_begin, pad;
+  int64_t a_pl, b_pl;   // workspace byte offsets of the A and B planes
+  double alpha, beta;
+};
+
+struct FmSplitArgs {
+  FmBases bases;
+  const FmSplitDesc* descs;
+  const FmSplitPrep* preps;
+  int32_t ndesc, total_tiles;
+  int32_t uniform, grid;        // as in FmGemmArgs
+  int32_t nprep;
+  int32_t prep_bx, prep_by;     // prep grid: row blocks of 64, K slices of 32
+  int32_t pad;
+};
+
+// the preparation launch, then the grouped product launch
+hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, hipStream_t s);

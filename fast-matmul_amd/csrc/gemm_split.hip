@@ -21,6 +21,8 @@
 
 typedef __bf16 fm_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float fm_f32x4 __attribute__((ext_vector_type(4)));
+// native 16-byte vector (register-resident in arrays, unlike the uint4 struct)
+typedef uint32_t fm_u32x4 __attribute__((ext_vector_type(4)));
 
 // round-to-nearest-even float -> bf16 (inputs are finite)
 __device__ __forceinline__ uint16_t fm_bf16_bits(float x) {
@@ -229,6 +231,222 @@ hipError_t fm_gemm_launch_bf16x3(int cfg, const FmGemmArgs& a, hipStream_t s) {
     default:
       hipLaunchKernelGGL((fm_gemm_bf16x3_kernel<128, 128, 2, 2>), dim3(grid),
                          dim3(256), 0, s, a);
+      break;
+  }
+  return hipGetLastError();
+}
+
+// ---- slice planes prepared once per operand -------------------------------------
+// One work group cuts 64 rows x one 32-deep K slice of one operand: each
+// thread 8 consecutive k of one row.  Sources with rows contiguous (A) put
+// the 64 lanes of a wave on 64 rows (coalesced loads along the column);
+// sources with K contiguous (B) give each row 4 lanes (32-byte runs).  Each
+// thread writes its 8 k of the three planes (3 x 16 bytes).
+__global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first) {
+  const FmSplitPrep d = args.preps[first + blockIdx.z];
+  const int r0 = blockIdx.x * 64, kc = blockIdx.y;
+  if (r0 >= d.rows_pad || kc * 32 >= d.Kp) return;
+  const float* __restrict__ src =
+      reinterpret_cast<const float*>(fm_base(args.bases, d.src.role)) + d.src.off;
+  char* ws = reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]);
+  const int tid = threadIdx.x;
+  const int r = d.kcontig ? tid >> 2 : tid & 63;
+  const int q = d.kcontig ? tid & 3 : tid >> 6;
+  const int row = r0 + r, k = kc * 32 + 8 * q;
+  const bool rok = row < d.rows;
+  const int64_t ld = d.src.ld;
+  float x[8];
+  if (d.kcontig) {
+    const float* p = src + (int64_t)row * ld + k;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = (rok && k + e < d.K) ? p[e] : 0.f;
+  } else {
+    const float* p = src + row + (int64_t)k * ld;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = (rok && k + e < d.K) ? p[(int64_t)e * ld] : 0.f;
+  }
+  uint4 sl[3];
+  fm_split3(x, sl);
+  char* dst = ws + d.dst + ((int64_t)row * (d.Kp >> 5) + kc) * 192 + q * 16;
+#pragma unroll
+  for (int p = 0; p < 3; ++p) *reinterpret_cast<uint4*>(dst + p * 64) = sl[p];
+}
+
+// ---- grouped products on slice planes -------------------------------------------
+// K in 32-deep slices; each slice of a tile is BM (A) + BN (B) rows of 192
+// bytes, copied from the planes into LDS with 16-byte loads (no bounds
+// checks: the planes are padded to whole tiles and K slices).  LDS is
+// double-buffered: the loads of slice kt + 1 are in flight while slice kt
+// is multiplied, then stored to the other buffer; one barrier per slice.
+// With DB false the LDS holds one slice (two work groups fit on a CU): the
+// next slice waits in registers and is stored after a second barrier.
+// In LDS each plane row is 64 bytes (4 chunks of 8 bf16); chunk c of row r
+// sits at position c ^ ((r >> 2) & 3), so the 16 lanes of a fragment read
+// (rows r..r+15, one chunk) cover all banks once.
+// The slice products and their order per output fragment are those of
+// fm_gemm_bf16x3_kernel, so both kernels give the same bits.
+template <int BM, int BN, int WM, int WN, bool DB>
+__global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int WTM = BM / WM, WTN = BN / WN;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  constexpr int A_CH = BM * 12 / NT, B_CH = BN * 12 / NT;  // 16-byte chunks per thread and slice
+  static_assert(A_CH * NT == BM * 12 && B_CH * NT == BN * 12, "tile/thread mismatch");
+  static_assert(TM >= 1 && TN >= 1, "bad tile");
+
+  constexpr int NBUF = DB ? 2 : 1;
+  __shared__ __align__(16) fm_u32x4 As[NBUF][3][BM][4];
+  __shared__ __align__(16) fm_u32x4 Bs[NBUF][3][BN][4];
+
+  for (int tile = blockIdx.x; tile < args.total_tiles; tile += gridDim.x) {
+  const int wg = fm_xcd_remap2(tile, args.total_tiles);
+  int g;
+  if (args.uniform) {
+    g = wg / (args.descs[0].tiles_m * args.descs[0].tiles_n);
+  } else {
+    int lo = 0, hi = args.ndesc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (args.descs[mid].tile_begin <= wg)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    g = lo;
+  }
+  const FmSplitDesc d = args.descs[g];
+  const int local = wg - d.tile_begin;
+  const int m0 = (local % d.tiles_m) * BM, n0 = (local / d.tiles_m) * BN;
+  const int nkc = d.Kp >> 5;
+  const char* ws = reinterpret_cast<const char*>(args.bases.p[FM_ROLE_WS]);
+  // chunk o of a slice: plane o / (4 * rows), row (o / 4) % rows, chunk o % 4
+  const fm_u32x4* Ap = reinterpret_cast<const fm_u32x4*>(ws + d.a_pl) + (int64_t)m0 * nkc * 12;
+  const fm_u32x4* Bp = reinterpret_cast<const fm_u32x4*>(ws + d.b_pl) + (int64_t)n0 * nkc * 12;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+
+  fm_u32x4 ra[A_CH], rb[B_CH];
+  auto gload = [&](int kt) {
+#pragma unroll
+    for (int u = 0; u < A_CH; ++u) {
+      const int o = tid + u * NT, c = o & 3, row = (o >> 2) % BM, p = (o >> 2) / BM;
+      ra[u] = Ap[((int64_t)row * nkc + kt) * 12 + p * 4 + c];
+    }
+#pragma unroll
+    for (int u = 0; u < B_CH; ++u) {
+      const int o = tid + u * NT, c = o & 3, col = (o >> 2) % BN, p = (o >> 2) / BN;
+      rb[u] = Bp[((int64_t)col * nkc + kt) * 12 + p * 4 + c];
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < A_CH; ++u) {
+      const int o = tid + u * NT, c = o & 3, row = (o >> 2) % BM, p = (o >> 2) / BM;
+      As[buf][p][row][c ^ ((row >> 2) & 3)] = ra[u];
+    }
+#pragma unroll
+    for (int u = 0; u < B_CH; ++u) {
+      const int o = tid + u * NT, c = o & 3, col = (o >> 2) % BN, p = (o >> 2) / BN;
+      Bs[buf][p][col][c ^ ((col >> 2) & 3)] = rb[u];
+    }
+  };
+
+  fm_f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = fm_f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int fr = lane & 15, fc = lane >> 4;
+  const int fsw = fc ^ ((fr >> 2) & 3);  // rows of a fragment start at multiples of 16
+  if (nkc > 0) {
+    gload(0);
+    sstore(0);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nkc; ++kt) {
+    const int buf = DB ? kt & 1 : 0;
+    if (kt + 1 < nkc) gload(kt + 1);
+    fm_bf16x8 af[3][TM], bf[3][TN];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+        af[p][i] = *reinterpret_cast<const fm_bf16x8*>(&As[buf][p][wm * WTM + i * 16 + fr][fsw]);
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        bf[p][j] = *reinterpret_cast<const fm_bf16x8*>(&Bs[buf][p][wn * WTN + j * 16 + fr][fsw]);
+    }
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        fm_f32x4 c = acc[i][j];
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[2][j], af[0][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[1][j], af[1][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[2][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[1][j], af[0][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[1][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[0][i], c, 0, 0, 0);
+        acc[i][j] = c;
+      }
+    if (DB) {
+      if (kt + 1 < nkc) sstore(buf ^ 1);
+      __syncthreads();
+    } else if (kt + 1 < nkc) {
+      __syncthreads();
+      sstore(0);
+      __syncthreads();
+    }
+  }
+
+  const float alpha = (float)d.alpha, beta = (float)d.beta;
+  float* __restrict__ C = reinterpret_cast<float*>(fm_base(args.bases, d.c.role)) + d.c.off;
+  const int64_t ldc = d.c.ld;
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int row = m0 + wm * WTM + i * 16 + fr;
+    if (row >= d.M) continue;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = n0 + wn * WTN + j * 16 + 4 * fc + v;
+        if (col < d.N) {
+          float* p = C + row + (int64_t)col * ldc;
+          float r = alpha * acc[i][j][v];
+          if (beta != 0.f) r += beta * *p;
+          *p = r;
+        }
+      }
+    }
+  }
+  }  // tile loop
+}
+
+hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, hipStream_t s) {
+  if (a.nprep > 0 && a.prep_bx > 0 && a.prep_by > 0) {
+    for (int first = 0; first < a.nprep; first += 65535) {
+      const int nz = a.nprep - first < 65535 ? a.nprep - first : 65535;
+      hipLaunchKernelGGL(fm_split_prep, dim3(a.prep_bx, a.prep_by, nz), dim3(256), 0, s, a, first);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (a.total_tiles <= 0) return hipSuccess;
+  const int grid = a.grid > 0 && a.grid < a.total_tiles ? a.grid : a.total_tiles;
+  switch (cfg) {
+    case FM_TILE_SP_256x128:
+      hipLaunchKernelGGL((fm_split_gemm<256, 128, 4, 2, true>), dim3(grid), dim3(512), 0, s, a);
+      break;
+    case FM_TILE_SP_128x256:
+      hipLaunchKernelGGL((fm_split_gemm<128, 256, 2, 4, true>), dim3(grid), dim3(512), 0, s, a);
+      break;
+    case FM_TILE_SP_128x128_DB:
+      hipLaunchKernelGGL((fm_split_gemm<128, 128, 2, 2, true>), dim3(grid), dim3(256), 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((fm_split_gemm<128, 128, 2, 2, false>), dim3(grid), dim3(256), 0, s, a);
       break;
   }
   return hipGetLastError();

@@ -24,6 +24,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
+#include <tuple>
 
 namespace fmplan {
 
@@ -219,6 +221,70 @@ fm_status ozaki_launch(int S, const std::vector<FmGemmDesc>& prods, size_t* ws_b
   return FM_OK;
 }
 
+// Split-bf16 leaf products on slice planes: one preparation record per
+// distinct operand (products that share an operand share its planes), the
+// planes appended to the workspace at byte offset *ws_bytes (advanced).
+Launch split_launch(std::vector<FmGemmDesc> prods, int cfg, size_t* ws_bytes) {
+  int bm, bn;
+  fm_tile_dims(cfg, &bm, &bn);
+  Launch L;
+  L.kind = LK_SPLIT;
+  L.stage = FM_STAGE_LEAF;
+  L.cfg = cfg;
+  size_t off = (size_t)align_up((int64_t)*ws_bytes, 256);
+  std::map<std::tuple<int, int64_t, int64_t, int, int, int>, int64_t> seen;
+  int max_rb = 0, max_kc = 0;
+  auto planes = [&](const FmOp& src, int rows, int K, int pad_to, int kcontig) {
+    const auto key = std::make_tuple(src.role, src.off, src.ld, rows, K, kcontig);
+    auto it = seen.find(key);
+    if (it != seen.end()) return it->second;
+    FmSplitPrep r;
+    memset(&r, 0, sizeof(r));
+    r.src = src;
+    r.rows = rows;
+    r.K = K;
+    r.rows_pad = (int32_t)align_up(rows, pad_to);
+    r.Kp = (int32_t)align_up(K, 32);
+    r.kcontig = kcontig;
+    r.dst = (int64_t)off;
+    off += (size_t)align_up((int64_t)r.rows_pad * r.Kp * 6, 256);
+    max_rb = std::max(max_rb, r.rows_pad / 64);
+    max_kc = std::max(max_kc, r.Kp / 32);
+    L.spreps.push_back(r);
+    seen.emplace(key, r.dst);
+    return r.dst;
+  };
+  int32_t t = 0;
+  bool uniform = true;
+  for (const FmGemmDesc& g : prods) {
+    FmSplitDesc d;
+    memset(&d, 0, sizeof(d));
+    d.c = g.c;
+    d.M = g.M;
+    d.N = g.N;
+    d.K = g.K;
+    d.Kp = (int32_t)align_up(g.K, 32);
+    d.tiles_m = (g.M + bm - 1) / bm;
+    d.tiles_n = (g.N + bn - 1) / bn;
+    d.tile_begin = t;
+    t += d.tiles_m * d.tiles_n;
+    d.alpha = g.alpha;
+    d.beta = g.beta;
+    d.a_pl = planes(g.a, g.M, g.K, bm, 0);
+    d.b_pl = planes(g.b, g.N, g.K, bn, 1);
+    uniform = uniform && g.M == prods[0].M && g.N == prods[0].N;
+    L.sdescs.push_back(d);
+  }
+  L.sp.ndesc = (int32_t)L.sdescs.size();
+  L.sp.total_tiles = t;
+  L.sp.uniform = uniform ? 1 : 0;
+  L.sp.nprep = (int32_t)L.spreps.size();
+  L.sp.prep_bx = max_rb;
+  L.sp.prep_by = max_kc;
+  *ws_bytes = off;
+  return L;
+}
+
 }  // namespace
 
 int g_tile_override = -1;
@@ -275,8 +341,12 @@ const TileCand kF32Tiles[] = {
     {FM_TILE_64x128, FM_TILE_64x128, 0.74, 3},
     {FM_TILE_128x64, FM_TILE_128x64, 0.71, 3},
     {FM_TILE_64x64, FM_TILE_64x64, 0.66, 4}};
-const TileCand kSplitTiles[] = {{FM_TILE_128x128, FM_TILE_128x128, 1.00, 2},
-                                {FM_TILE_64x64, FM_TILE_64x64, 0.80, 4}};
+// split-bf16: kernels on prepared slice planes (any shape: the planes are
+// padded to whole tiles)
+const TileCand kSplitTiles[] = {{FM_TILE_SP_256x128, FM_TILE_SP_256x128, 1.00, 1},
+                                {FM_TILE_SP_128x256, FM_TILE_SP_128x256, 1.00, 1},
+                                {FM_TILE_SP_128x128, FM_TILE_SP_128x128, 0.95, 2},
+                                {FM_TILE_SP_128x128_DB, FM_TILE_SP_128x128_DB, 0.90, 1}};
 // edge kernels run slower than whole-tile ones (predicated loads)
 const double kEdgePenalty = 0.93;
 
@@ -304,6 +374,7 @@ int64_t count_tiles(int cfg, const std::vector<FmGemmDesc>& prods, size_t from =
 
 // Every product a whole number of tiles and K slices, columns 16-byte aligned.
 bool whole_tiles(fm_dtype dt, int cfg, const std::vector<FmGemmDesc>& prods) {
+  if (fm_tile_planes(cfg)) return true;
   int bm, bn;
   fm_tile_dims(cfg, &bm, &bn);
   const int bk = fm_tile_bk(cfg), vl = dt == FM_F64 ? 2 : 4;
@@ -644,13 +715,20 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
       P.launches.push_back(std::move(L));
     } else {
       const int cfg = pick_tile(P.dtype, P.mode, leaves);
+      if (fm_tile_planes(cfg) && P.mode != FM_MODE_SPLIT_BF16)
+        return set_err(FM_ERR_INVALID_ARG, "tile config " + std::to_string(cfg) + " is for split_bf16 leaves");
       std::vector<FmGemmDesc> rest;
       int cfg_rest = cfg, cfg_head = cfg;
       if (persistent_grid == 0 && g_split_remainder)
         split_remainder(P, cfg, leaves, rest, &cfg_rest, &cfg_head);
-      P.launches.push_back(gemm_launch(std::move(leaves), cfg_head, FM_STAGE_LEAF, P.dtype));
+      auto leaf = [&](std::vector<FmGemmDesc> prods, int c) {
+        if (fm_tile_planes(c)) return split_launch(std::move(prods), c, &wsb);
+        return gemm_launch(std::move(prods), c, FM_STAGE_LEAF, P.dtype);
+      };
+      P.launches.push_back(leaf(std::move(leaves), cfg_head));
       P.launches.back().gm.grid = persistent_grid;
-      if (!rest.empty()) P.launches.push_back(gemm_launch(std::move(rest), cfg_rest, FM_STAGE_LEAF, P.dtype));
+      P.launches.back().sp.grid = persistent_grid;
+      if (!rest.empty()) P.launches.push_back(leaf(std::move(rest), cfg_rest));
     }
   }
   int maxd = 0;
@@ -687,7 +765,10 @@ fm_status Plan::upload() {
       L.dev_off = put(L.gdescs.data(), L.gdescs.size() * sizeof(FmGemmDesc));
     else if (L.kind == LK_OZAKI)
       L.dev_off = put(L.odescs.data(), L.odescs.size() * sizeof(FmOzakiDesc));
-    else
+    else if (L.kind == LK_SPLIT) {
+      L.dev_off = put(L.sdescs.data(), L.sdescs.size() * sizeof(FmSplitDesc));
+      L.dev_off2 = put(L.spreps.data(), L.spreps.size() * sizeof(FmSplitPrep));
+    } else
       L.dev_off = put(L.ops.data(), L.ops.size() * sizeof(FmOp));
   }
   ext_off = put(ext.data(), ext.size() * sizeof(void*));
@@ -705,7 +786,10 @@ fm_status Plan::upload() {
       L.gm.descs = reinterpret_cast<const FmGemmDesc*>(p);
     else if (L.kind == LK_OZAKI)
       L.oz.descs = reinterpret_cast<const FmOzakiDesc*>(p);
-    else
+    else if (L.kind == LK_SPLIT) {
+      L.sp.descs = reinterpret_cast<const FmSplitDesc*>(p);
+      L.sp.preps = reinterpret_cast<const FmSplitPrep*>((char*)dbuf + L.dev_off2);
+    } else
       L.st.ops = reinterpret_cast<const FmOp*>(p);
   }
   return FM_OK;
@@ -771,10 +855,16 @@ fm_status Plan::run(const void* A, const void* B, void* C, void* ws, hipStream_t
       pr.e1 = pool_event();
       if (pr.e0) (void)hipEventRecord(pr.e0, s);
     }
+    if (L.kind == LK_SPLIT && ((uintptr_t)ws % 16 != 0))
+      return set_err(FM_ERR_INVALID_ARG, "split_bf16 leaves need a 16-byte aligned workspace");
     if (L.kind == LK_OZAKI) {
       FmOzakiArgs o = L.oz;
       o.bases = bases;
       e = fm_ozaki_launch(L.oz_slices, o, L.oz_rows, L.oz_cols, s);
+    } else if (L.kind == LK_SPLIT) {
+      FmSplitArgs a = L.sp;
+      a.bases = bases;
+      e = fm_split_launch(L.cfg, a, s);
     } else if (L.kind == LK_GEMM) {
       FmGemmArgs g = L.gm;
       g.bases = bases;

@@ -19,7 +19,7 @@ fm_status hip_err(hipError_t e, const char* what);
 const std::string& last_error();
 void clear_error();
 
-enum LaunchKind { LK_S, LK_T, LK_W, LK_GEMM, LK_OZAKI, LK_S2, LK_T2, LK_W2, LK_NONE };
+enum LaunchKind { LK_S, LK_T, LK_W, LK_GEMM, LK_OZAKI, LK_S2, LK_T2, LK_W2, LK_SPLIT, LK_NONE };
 
 // One kernel launch.  Descriptors stay on the host until Plan::upload packs
 // them into the plan's device buffer.
@@ -30,13 +30,16 @@ struct Launch {
   FmStageArgs st{};
   FmGemmArgs gm{};
   FmOzakiArgs oz{};
+  FmSplitArgs sp{};
   int cfg = 0;
   int vec_ok = 1;
   int oz_slices = 0, oz_rows = 0, oz_cols = 0;
   std::vector<FmOp> ops;
   std::vector<FmGemmDesc> gdescs;
   std::vector<FmOzakiDesc> odescs;
-  size_t dev_off = 0;
+  std::vector<FmSplitDesc> sdescs;
+  std::vector<FmSplitPrep> spreps;
+  size_t dev_off = 0, dev_off2 = 0;
 };
 
 struct Plan {

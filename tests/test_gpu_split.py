@@ -98,3 +98,65 @@ def test_split_accuracy_fp32_level(fm, oracle_mod, dist, alg, steps):
     nw = np.abs(C[rows, cols].astype(np.float64) - D).max() / (np.abs(A).max() * np.abs(B).max() * k)
     nwp = np.abs(Cp[rows, cols].astype(np.float64) - D).max() / (np.abs(A).max() * np.abs(B).max() * k)
     assert nw <= 4 * nwp + 2.0 ** -24
+
+
+# FmTileCfg: 0 = in-loop split kernel (128 x 128), 24-27 = kernels on
+# prepared slice planes
+INLOOP, PLANE_CFGS = 0, (24, 25, 26, 27)
+
+
+@pytest.fixture
+def _tile_reset(fm):
+    yield
+    fm.set_gemm_tile(-1)
+
+
+@pytest.mark.parametrize("cfg", PLANE_CFGS)
+def test_plane_kernels_match_inloop_kernel_bits(fm, _tile_reset, cfg):
+    """Both kernels form the same six slice products per 32-deep K slice in
+    the same order, so they must agree bit for bit, over ragged shapes
+    (partial tiles, K not a multiple of 32) and with alpha / beta."""
+    for (m, k, n) in [(1, 1, 1), (17, 33, 65), (300, 500, 100), (515, 1031, 263),
+                      (256, 64, 384), (129, 2048, 255)]:
+        A, B, C0 = make_inputs(m, k, n, 41, "u11", np.float32, with_c=True)
+        Ad, Bd = _dev(A, fm), _dev(B, fm)
+        fm.set_gemm_tile(INLOOP)
+        ref = fm.gemm(Ad, Bd, _dev(C0, fm), 1.5, -0.5, mode="split_bf16").cpu().numpy()
+        fm.set_gemm_tile(cfg)
+        got = fm.gemm(Ad, Bd, _dev(C0, fm), 1.5, -0.5, mode="split_bf16").cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (m, k, n)
+
+
+@pytest.mark.parametrize("alg,steps,shape", [("strassen", 2, (1024, 1024, 1024)),
+                                             ("strassen", 2, (1000, 515, 777)),
+                                             ("grey424_26_206", 1, (512, 256, 512)),
+                                             ("smirnov333_23_139", 1, (384, 384, 384))])
+def test_fast_split_planes_match_inloop_bits(fm, _tile_reset, alg, steps, shape):
+    """A whole fast multiply with plane leaves equals the one with in-loop
+    leaves bit for bit (stages, peeling and W are shared)."""
+    m, k, n = shape
+    A, B, _ = make_inputs(m, k, n, 43, "u11", np.float32)
+    Ad, Bd = _dev(A, fm), _dev(B, fm)
+    fm.set_gemm_tile(INLOOP)
+    ref = fm.fastmatmul(alg, Ad, Bd, None, steps, mode="split_bf16").cpu().numpy()
+    fm.set_gemm_tile(-1)
+    info = fm.plan_info(alg, m, k, n, steps, dtype=torch.float32, mode="split_bf16")
+    assert all(l["cfg"] in PLANE_CFGS for l in info["leaf"])
+    got = fm.fastmatmul(alg, Ad, Bd, None, steps, mode="split_bf16").cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_split_planes_shared_operand(fm, _tile_reset):
+    """classical222_8_24 uses each block of A and of B in two products: each
+    is prepared once (4 + 4 plane sets, not 16) and both products read it."""
+    m = k = n = 512
+    info = fm.plan_info("classical222_8_24", m, k, n, 1, dtype=torch.float32, mode="split_bf16")
+    plain = fm.plan_info("classical222_8_24", m, k, n, 1, dtype=torch.float32)
+    assert info["workspace_bytes"] - plain["workspace_bytes"] == 8 * 256 * 256 * 6
+    A, B, _ = make_inputs(m, k, n, 47, "u11", np.float32)
+    Ad, Bd = _dev(A, fm), _dev(B, fm)
+    fm.set_gemm_tile(INLOOP)
+    ref = fm.fastmatmul("classical222_8_24", Ad, Bd, None, 1, mode="split_bf16").cpu().numpy()
+    fm.set_gemm_tile(-1)
+    got = fm.fastmatmul("classical222_8_24", Ad, Bd, None, 1, mode="split_bf16").cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))

@@ -81,6 +81,36 @@ def test_remainder_split_modes():
     assert p["launches"]["LEAF"] == 1 and p["leaf"][0]["cfg"] == -1
 
 
+def test_split_leaves_on_slice_planes():
+    """split_bf16 leaves run on bf16 slice planes prepared in the workspace:
+    per distinct operand rows_pad x Kp x 3 planes x 2 bytes (rows padded to
+    the tile, K to 32), on top of the plain FP32 plan's workspace."""
+    p = fastmm.plan_info("strassen", 4096, 4096, 4096, 1, dtype=torch.float32, mode="split_bf16")
+    q = fastmm.plan_info("strassen", 4096, 4096, 4096, 1, dtype=torch.float32)
+    assert p["leaf"][0]["cfg"] in (24, 25, 26, 27)
+    # Strassen level 1: 7 products of 2048^3, each with its own A-side
+    # operand (5 sums, the blocks A11 and A22) and B-side operand: 14 planes
+    planes = p["workspace_bytes"] - q["workspace_bytes"]
+    assert planes == 14 * 2048 * 2048 * 6
+    # ragged shapes: rows padded to the tile, K to 32
+    fastmm.set_gemm_tile(24)
+    try:
+        p = fastmm.plan_info("strassen", 300, 100, 200, 0, dtype=torch.float32, mode="split_bf16")
+    finally:
+        fastmm.set_gemm_tile(-1)
+    assert p["leaf"] == [{"cfg": 24, "products": 1, "tiles": 3 * 2}]
+    assert p["workspace_bytes"] == 384 * 128 * 6 + 256 * 128 * 6
+
+
+def test_plane_tiles_need_split_mode():
+    fastmm.set_gemm_tile(25)
+    try:
+        with pytest.raises(ValueError):
+            fastmm.plan_info("strassen", 1024, 1024, 1024, 1, dtype=torch.float32)
+    finally:
+        fastmm.set_gemm_tile(-1)
+
+
 def test_stage_bytes_match_closed_forms():
     """Algorithmic bytes per stage, grey424_26_206 L=2 FP64 8192^3: one-level
     S = 8 (blocks of level 1 + 26 x blocks of level 2) x (8 reads + 22 writes),
