@@ -281,10 +281,14 @@ __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first
 // With DB false the LDS holds one slice (two work groups fit on a CU): the
 // next slice waits in registers and is stored after a second barrier.
 // In LDS each plane row is 64 bytes (4 chunks of 8 bf16); chunk c of row r
-// sits at position c ^ ((r >> 2) & 3), so the 16 lanes of a fragment read
-// (rows r..r+15, one chunk) cover all banks once.
+// sits at position c ^ s(r), s(r) = ((r >> 2) & 3) ^ (r & 2), so the 16-byte
+// reads of a fragment (rows r..r+15 from a multiple of 16, one chunk) fall
+// in distinct banks for every 8 lanes (128 bytes) and every 16 lanes
+// (256 bytes), and so do the stores (4 chunks of 4 consecutive rows).
 // The slice products and their order per output fragment are those of
 // fm_gemm_bf16x3_kernel, so both kernels give the same bits.
+__device__ __forceinline__ int fm_sp_swz(int r) { return ((r >> 2) & 3) ^ (r & 2); }
+
 template <int BM, int BN, int WM, int WN, bool DB>
 __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) {
   constexpr int NT = 64 * WM * WN;
@@ -315,8 +319,16 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
     g = lo;
   }
   const FmSplitDesc d = args.descs[g];
+  // tiles of a product in groups of SP_GROUP tile rows: consecutive tiles go
+  // down the rows of a group, then across its columns, so the 32 to 64 work
+  // groups that run together on one XCD cover a square of about 1024 x 1024
+  // outputs and share its A and B panels in the XCD's L2
+  constexpr int SP_GROUP = BM == 256 ? 4 : 8;
   const int local = wg - d.tile_begin;
-  const int m0 = (local % d.tiles_m) * BM, n0 = (local / d.tiles_m) * BN;
+  const int per_group = SP_GROUP * d.tiles_n, grp = local / per_group;
+  const int first = grp * SP_GROUP, rows_g = min(SP_GROUP, d.tiles_m - first);
+  const int rem = local - grp * per_group;
+  const int m0 = (first + rem % rows_g) * BM, n0 = (rem / rows_g) * BN;
   const int nkc = d.Kp >> 5;
   const char* ws = reinterpret_cast<const char*>(args.bases.p[FM_ROLE_WS]);
   // chunk o of a slice: plane o / (4 * rows), row (o / 4) % rows, chunk o % 4
@@ -342,12 +354,12 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
 #pragma unroll
     for (int u = 0; u < A_CH; ++u) {
       const int o = tid + u * NT, c = o & 3, row = (o >> 2) % BM, p = (o >> 2) / BM;
-      As[buf][p][row][c ^ ((row >> 2) & 3)] = ra[u];
+      As[buf][p][row][c ^ fm_sp_swz(row)] = ra[u];
     }
 #pragma unroll
     for (int u = 0; u < B_CH; ++u) {
       const int o = tid + u * NT, c = o & 3, col = (o >> 2) % BN, p = (o >> 2) / BN;
-      Bs[buf][p][col][c ^ ((col >> 2) & 3)] = rb[u];
+      Bs[buf][p][col][c ^ fm_sp_swz(col)] = rb[u];
     }
   };
 
@@ -358,7 +370,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
     for (int j = 0; j < TN; ++j) acc[i][j] = fm_f32x4{0.f, 0.f, 0.f, 0.f};
 
   const int fr = lane & 15, fc = lane >> 4;
-  const int fsw = fc ^ ((fr >> 2) & 3);  // rows of a fragment start at multiples of 16
+  const int fsw = fc ^ fm_sp_swz(fr);  // rows of a fragment start at multiples of 16
   if (nkc > 0) {
     gload(0);
     sstore(0);
