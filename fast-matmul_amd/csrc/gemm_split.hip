@@ -281,13 +281,15 @@ __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first
 // With DB false the LDS holds one slice (two work groups fit on a CU): the
 // next slice waits in registers and is stored after a second barrier.
 // In LDS each plane row is 64 bytes (4 chunks of 8 bf16); chunk c of row r
-// sits at position c ^ s(r), s(r) = ((r >> 2) & 3) ^ (r & 2), so the 16-byte
-// reads of a fragment (rows r..r+15 from a multiple of 16, one chunk) fall
-// in distinct banks for every 8 lanes (128 bytes) and every 16 lanes
-// (256 bytes), and so do the stores (4 chunks of 4 consecutive rows).
+// sits at position c ^ ((r >> 1) & 3).  Measured on MI355X (SQ_LDS_BANK_CONFLICT
+// over the fragment read pattern: rows r..r+15 from a multiple of 16, one
+// chunk per 16 lanes) this is the swizzle without bank conflicts: the plain
+// layout, 80-byte padded rows and swizzles that also mix in row bit 3 all
+// conflict and read at about half the rate.  The stores (4 chunks of 16
+// consecutive rows per wave) are conflict-free with it too.
 // The slice products and their order per output fragment are those of
 // fm_gemm_bf16x3_kernel, so both kernels give the same bits.
-__device__ __forceinline__ int fm_sp_swz(int r) { return ((r >> 2) & 3) ^ (r & 2); }
+__device__ __forceinline__ int fm_sp_swz(int r) { return (r >> 1) & 3; }
 
 template <int BM, int BN, int WM, int WN, bool DB>
 __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) {
