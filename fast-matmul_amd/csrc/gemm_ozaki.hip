@@ -13,8 +13,13 @@
 // (S+1) 2^-7S relative to |A[i,:]|max |B[:,j]|max per term.
 //
 // Two preparation kernels write the digit planes once per operand into the
-// workspace (A: K contiguous per row, B: K contiguous per column, K padded
-// to 64), then one grouped kernel runs every leaf product of the launch.
+// workspace, then one grouped kernel runs every leaf product of the launch.
+// Digit layout of an operand with rows_pad rows (A: rows, B: columns), K
+// padded to 64: digit s of (row r, k) is the byte at
+//   slc + ((k / 32 * 2 S + 2 s + (k / 16) % 2) * rows_pad + r) * 16 + k % 16,
+// i.e. for every 16-deep half of a 32-deep K slice and every plane, the
+// 16-byte digit groups of all rows are consecutive: the preparation writes
+// and the product kernel's loads and LDS stores all move 1 KB runs per wave.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -30,21 +35,36 @@ __device__ __forceinline__ int fm_exp_bound(double mx) {
   return e;
 }
 
-// Digits of y (|y| < 1): S truncated 7-bit slices, each in [-127, 127].
+// Digits of y (|y| < 1): S truncated 7-bit slices, each in [-127, 127]:
+// d_s = sign(y) * (floor(|y| 2^(7 s)) mod 128), s = 1..S (S <= 8), the
+// digits that repeated truncation (y <- 128 y - trunc(128 y)) would give.
+// They are read as bit fields of h = floor(|y| 2^28) and
+// l = floor((|y| 2^28 - h) 2^28), both exact (power-of-two scalings and
+// the difference of a value and its integer part).
 template <int S>
 __device__ __forceinline__ void fm_digits(double y, int8_t (&d)[S]) {
+  static_assert(S <= 8, "at most 56 digit bits");
+  const double a = fabs(y) * 268435456.0;
+  const double h = trunc(a);
+  const uint32_t hi = (uint32_t)h;
+  const uint32_t lo = S > 4 ? (uint32_t)trunc((a - h) * 268435456.0) : 0u;
+  const bool neg = y < 0.0;
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    y = y * 128.0;               // exact: power-of-two scaling
-    const double t = trunc(y);   // |t| <= 127 since |y| < 128
-    d[s] = (int8_t)(int)t;
-    y = y - t;                   // exact: t holds y's leading bits
+    const uint32_t v = s < 4 ? (hi >> (21 - 7 * s)) & 127u : (lo >> (21 - 7 * (s - 4))) & 127u;
+    d[s] = (int8_t)(neg ? -(int)v : (int)v);
   }
 }
 
+// byte offset of the 16-byte digit group of (row r, k .. k + 15), plane s
+// (k a multiple of 16)
+template <int S>
+__host__ __device__ __forceinline__ int64_t fm_oz_chunk(int k, int s, int rows_pad, int r) {
+  return ((int64_t)((k >> 5) * 2 * S + 2 * s + ((k >> 4) & 1)) * rows_pad + r) * 16;
+}
+
 // ---- A side: one work group per 64 rows of one operand ---------------------
-// Row exponents to ws + exp_off (int32), digits to ws + slc_off:
-// plane s, row r, byte k at slc_off + (s * rows_pad + r) * kp + k.
+// Row exponents to ws + a_exp (int32), digits to ws + a_slc.
 template <int S>
 __global__ void __launch_bounds__(256) fm_ozaki_prep_rows(FmOzakiArgs args) {
   const FmOzakiDesc d = args.descs[blockIdx.y];
@@ -60,8 +80,10 @@ __global__ void __launch_bounds__(256) fm_ozaki_prep_rows(FmOzakiArgs args) {
   const bool rok = row < d.M;
   __shared__ double red[4][64];
   double mx = 0.0;
-  if (rok)
+  if (rok) {
+#pragma unroll 8
     for (int k = kq; k < d.K; k += 4) mx = fmax(mx, fabs(A[row + (int64_t)k * lda]));
+  }
   red[kq][r] = mx;
   __syncthreads();
   mx = fmax(fmax(red[0][r], red[1][r]), fmax(red[2][r], red[3][r]));
@@ -80,41 +102,61 @@ __global__ void __launch_bounds__(256) fm_ozaki_prep_rows(FmOzakiArgs args) {
       for (int s = 0; s < S; ++s) dig[s][q] = t[s];
     }
     if (row < d.rows_pad) {
+      const int k = k0 + 16 * kq;
 #pragma unroll
       for (int s = 0; s < S; ++s) {
-        int8_t* p = sl + ((int64_t)s * d.rows_pad + row) * d.Kp + k0 + 16 * kq;
+        int8_t* p = sl + fm_oz_chunk<S>(k, s, d.rows_pad, row);
         *reinterpret_cast<int4*>(p) = *reinterpret_cast<const int4*>(&dig[s][0]);
       }
     }
   }
 }
 
-// ---- B side: one wave per column ---------------------------------------------
-// plane t, column c, byte k at b_slc + (t * cols_pad + c) * kp + k.
+// ---- B side: one work group per 64 columns of one operand ------------------
+// Column exponents to ws + b_exp, digits to ws + b_slc.  The exponent of a
+// column is a reduction along its K contiguous entries (one wave per column,
+// coalesced); the digits are cut with the 64 lanes of a wave on 64 columns,
+// each lane 16 consecutive k of its column (one 128-byte line), so the digit
+// groups go out as consecutive 16-byte slots.
 template <int S>
 __global__ void __launch_bounds__(256) fm_ozaki_prep_cols(FmOzakiArgs args) {
   const FmOzakiDesc d = args.descs[blockIdx.y];
-  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (col >= d.cols_pad) return;
+  const int c0 = blockIdx.x * 64;
+  if (c0 >= d.cols_pad) return;
   const double* B = reinterpret_cast<const double*>(fm_base(args.bases, d.b.role)) + d.b.off;
+  const int64_t ldb = d.b.ld;
   char* ws = reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]);
   int32_t* ex = reinterpret_cast<int32_t*>(ws + d.b_exp);
   int8_t* sl = reinterpret_cast<int8_t*>(ws + d.b_slc);
-  const bool cok = col < d.N;
-  const double* bc = B + (int64_t)col * d.b.ld;
-  double mx = 0.0;
-  if (cok)
-    for (int k = lane; k < d.K; k += 64) mx = fmax(mx, fabs(bc[k]));
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ int es[64];
+  for (int q = 0; q < 16; ++q) {
+    const int cl = wave * 16 + q, col = c0 + cl;
+    double mx = 0.0;
+    if (col < d.N) {
+      const double* bc = B + (int64_t)col * ldb;
+#pragma unroll 8
+      for (int k = lane; k < d.K; k += 64) mx = fmax(mx, fabs(bc[k]));
+    }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
-  const int e = fm_exp_bound(mx);
-  if (lane == 0 && cok) ex[col] = e;
-  for (int k0 = 16 * lane; k0 < d.Kp; k0 += 16 * 64) {
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    const int e = fm_exp_bound(mx);
+    if (lane == 0) {
+      es[cl] = e;
+      if (col < d.N) ex[col] = e;
+    }
+  }
+  __syncthreads();
+  const int cl = tid & 63, kq = tid >> 6, col = c0 + cl;
+  const bool cok = col < d.N;
+  const int e = es[cl];
+  const double* bc = B + (int64_t)col * ldb;
+  for (int k0 = 0; k0 < d.Kp; k0 += 64) {
+    const int kb = k0 + 16 * kq;
     int8_t dig[S][16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int k = k0 + q;
+      const int k = kb + q;
       const double v = (cok && k < d.K) ? ldexp(bc[k], -e) : 0.0;
       int8_t t[S];
       fm_digits<S>(v, t);
@@ -123,7 +165,7 @@ __global__ void __launch_bounds__(256) fm_ozaki_prep_cols(FmOzakiArgs args) {
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      int8_t* p = sl + ((int64_t)s * d.cols_pad + col) * d.Kp + k0;
+      int8_t* p = sl + fm_oz_chunk<S>(kb, s, d.cols_pad, col);
       *reinterpret_cast<int4*>(p) = *reinterpret_cast<const int4*>(&dig[s][0]);
     }
   }
@@ -174,8 +216,8 @@ __global__ void __launch_bounds__(256) fm_ozaki_gemm(FmOzakiArgs args) {
   for (int k0 = 0; k0 < d.Kp; k0 += BK) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      const int4 va = *reinterpret_cast<const int4*>(sa + ((int64_t)s * d.rows_pad + m0 + lr) * d.Kp + k0 + lc);
-      const int4 vb = *reinterpret_cast<const int4*>(sb + ((int64_t)s * d.cols_pad + n0 + lr) * d.Kp + k0 + lc);
+      const int4 va = *reinterpret_cast<const int4*>(sa + fm_oz_chunk<S>(k0 + lc, s, d.rows_pad, m0 + lr));
+      const int4 vb = *reinterpret_cast<const int4*>(sb + fm_oz_chunk<S>(k0 + lc, s, d.cols_pad, n0 + lr));
       *reinterpret_cast<int4*>(&As[s][lr][lc]) = va;
       *reinterpret_cast<int4*>(&Bs[s][lr][lc]) = vb;
     }
@@ -234,13 +276,140 @@ __global__ void __launch_bounds__(256) fm_ozaki_gemm(FmOzakiArgs args) {
   }
 }
 
+// ---- grouped digit product, 32-deep K slices -----------------------------------
+// Work group: 2 x 2 waves, each a 32 x 32 block computed by
+// v_mfma_i32_32x32x32_i8 (64 x 64 output tile); K in 32-deep slices with
+// LDS double-buffered (all S planes of A and B, 2 x 32 KB for S = 8) so two
+// work groups share a CU, the next slice loaded into registers while the
+// current one is multiplied.  LDS keeps the digit groups in their global
+// order (plane half j = 2 s + h, then row), so the loads, the LDS stores
+// (lane l: row l) and the fragment reads (lane l: row l % 32, half l / 32)
+// all move consecutive 16-byte slots.  The tiles of a product go 8 tile
+// rows at a time, so the 64 work groups that run together on one XCD share
+// 512 rows of A and 512 columns of B in its L2.  The per-scale int32 sums
+// are exact, so the result equals fm_ozaki_gemm's bit for bit.
+typedef int fm_i32x16 __attribute__((ext_vector_type(16)));
+
+template <int S>
+__global__ void __launch_bounds__(256, 2) fm_ozaki_gemm32(FmOzakiArgs args) {
+  constexpr int BM = 64, BN = 64, J = 2 * S, GROUP = 8;
+  constexpr int NCH = (J * BM + 255) / 256;  // 16-byte chunks per thread and side per slice
+  __shared__ __align__(16) fm_i32x4 As[2][J][BM];
+  __shared__ __align__(16) fm_i32x4 Bs[2][J][BN];
+
+  int wg = blockIdx.x;
+  {
+    const int nwg = gridDim.x, full = nwg & ~7;
+    if (wg < full) wg = (wg & 7) * (full >> 3) + (wg >> 3);
+  }
+  int lo = 0, hi = args.ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (args.descs[mid].tile_begin <= wg)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const FmOzakiDesc d = args.descs[lo];
+  const int local = wg - d.tile_begin;
+  const int per_group = GROUP * d.tiles_n, grp = local / per_group;
+  const int first = grp * GROUP, rows_g = min(GROUP, d.tiles_m - first);
+  const int rem = local - grp * per_group;
+  const int m0 = (first + rem % rows_g) * BM, n0 = (rem / rows_g) * BN;
+  char* ws = reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]);
+  const fm_i32x4* sa = reinterpret_cast<const fm_i32x4*>(ws + d.a_slc) + m0;
+  const fm_i32x4* sb = reinterpret_cast<const fm_i32x4*>(ws + d.b_slc) + n0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int64_t pa = d.rows_pad, pb = d.cols_pad;
+
+  fm_i32x16 acc[S];  // one exact integer sum per scale group u = s + t + 2
+#pragma unroll
+  for (int u = 0; u < S; ++u)
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[u][e] = 0;
+
+  fm_i32x4 ra[NCH], rb[NCH];
+  auto gload = [&](int kc) {
+#pragma unroll
+    for (int u = 0; u < NCH; ++u) {
+      const int o = tid + u * 256, row = o & (BM - 1), j = o / BM;
+      if ((J * BM) % 256 == 0 || o < J * BM) {
+        ra[u] = sa[(int64_t)(kc * J + j) * pa + row];
+        rb[u] = sb[(int64_t)(kc * J + j) * pb + row];
+      }
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < NCH; ++u) {
+      const int o = tid + u * 256, row = o & (BM - 1), j = o / BM;
+      if ((J * BM) % 256 == 0 || o < J * BM) {
+        As[buf][j][row] = ra[u];
+        Bs[buf][j][row] = rb[u];
+      }
+    }
+  };
+
+  const int nkc = d.Kp >> 5;
+  const int fr = lane & 31, fh = lane >> 5;
+  if (nkc > 0) {
+    gload(0);
+    sstore(0);
+  }
+  __syncthreads();
+  for (int kc = 0; kc < nkc; ++kc) {
+    const int buf = kc & 1;
+    if (kc + 1 < nkc) gload(kc + 1);
+    fm_i32x4 af[S], bf[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      af[s] = As[buf][2 * s + fh][wm * 32 + fr];
+      bf[s] = Bs[buf][2 * s + fh][wn * 32 + fr];
+    }
+#pragma unroll
+    for (int s = 0; s < S; ++s)
+#pragma unroll
+      for (int t = 0; t + s < S; ++t)
+        acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[t], af[s], acc[s + t], 0, 0, 0);
+    if (kc + 1 < nkc) sstore(buf ^ 1);
+    __syncthreads();
+  }
+
+  // lane l holds C row (l % 32) of its block and, in element e, column
+  // (e % 4) + 4 (l / 32) + 8 (e / 4)
+  const int32_t* ea = reinterpret_cast<const int32_t*>(ws + d.a_exp);
+  const int32_t* eb = reinterpret_cast<const int32_t*>(ws + d.b_exp);
+  double* C = reinterpret_cast<double*>(fm_base(args.bases, d.c.role)) + d.c.off;
+  const int64_t ldc = d.c.ld;
+  const int row = m0 + wm * 32 + fr;
+  if (row >= d.M) return;
+  const int er = ea[row];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) {
+    const int col = n0 + wn * 32 + (e & 3) + 4 * fh + 8 * (e >> 2);
+    if (col >= d.N) continue;
+    // smallest scale first: group u = s + t + 2 carries 2^-7u
+    double sum = 0.0;
+#pragma unroll
+    for (int u = S - 1; u >= 0; --u) sum = __dadd_rn(sum, ldexp((double)acc[u][e], -7 * (u + 2)));
+    double r = __dmul_rn(ldexp(sum, er + eb[col]), d.alpha);
+    double* p = C + row + (int64_t)col * ldc;
+    if (d.beta != 0.0) r = __dadd_rn(r, __dmul_rn(d.beta, *p));
+    *p = r;
+  }
+}
+
 template <int S>
 static hipError_t fm_ozaki_run(const FmOzakiArgs& a, int max_rows, int max_cols, hipStream_t s) {
   hipLaunchKernelGGL((fm_ozaki_prep_rows<S>), dim3((max_rows + 63) / 64, a.ndesc), dim3(256), 0,
                      s, a);
-  hipLaunchKernelGGL((fm_ozaki_prep_cols<S>), dim3((max_cols + 3) / 4, a.ndesc), dim3(256), 0, s,
-                     a);
-  hipLaunchKernelGGL((fm_ozaki_gemm<S>), dim3(a.total_tiles), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((fm_ozaki_prep_cols<S>), dim3((max_cols + 63) / 64, a.ndesc), dim3(256), 0,
+                     s, a);
+  if (a.variant == 0)
+    hipLaunchKernelGGL((fm_ozaki_gemm<S>), dim3(a.total_tiles), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((fm_ozaki_gemm32<S>), dim3(a.total_tiles), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

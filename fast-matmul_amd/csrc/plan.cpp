@@ -214,6 +214,8 @@ fm_status ozaki_launch(int S, const std::vector<FmGemmDesc>& prods, size_t* ws_b
   L.stage = FM_STAGE_LEAF;
   L.oz.ndesc = (int32_t)L.odescs.size();
   L.oz.total_tiles = t;
+  // the 64 x 64 tile override selects the 16 x 16 x 64 kernel
+  L.oz.variant = g_tile_override == FM_TILE_64x64 ? 0 : 1;
   L.oz_slices = S;
   L.oz_rows = max_rows;
   L.oz_cols = max_cols;

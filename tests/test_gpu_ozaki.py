@@ -24,19 +24,25 @@ def _dev(a, fm):
     return fm.col_major(torch.from_numpy(np.asfortranarray(a)).to(DEV))
 
 
+# FmTileCfg 1 (64 x 64) selects the 16 x 16 x 64 digit kernel, -1 the
+# default 32 x 32 x 32 one
+@pytest.mark.parametrize("kernel", [-1, 1])
 @pytest.mark.parametrize("S", [6, 7, 8])
-@pytest.mark.parametrize("shape", [(64, 64, 64), (100, 130, 70), (257, 500, 55), (1, 64, 1)])
-def test_digit_gemm_matches_restatement(fm, S, shape):
+@pytest.mark.parametrize("shape", [(64, 64, 64), (100, 130, 70), (257, 500, 55), (1, 64, 1),
+                                   (130, 700, 96)])
+def test_digit_gemm_matches_restatement(fm, S, shape, kernel):
     m, k, n = shape
     A, B, C0 = make_inputs(m, k, n, S * 7 + m, "u11", with_c=True)
     A[3 % m, :] *= 1e-9  # rows/columns with very different scales
     B[:, 2 % n] *= 1e6
     fm.set_ozaki_slices(S)
+    fm.set_gemm_tile(kernel)
     try:
         C = fm.gemm(_dev(A, fm), _dev(B, fm), _dev(C0, fm), 1.5, -0.5,
                     mode="ozaki_int8").cpu().numpy()
     finally:
         fm.set_ozaki_slices(8)
+        fm.set_gemm_tile(-1)
     want = ozaki.ozaki_matmul(A, B, S, 1.5, -0.5, C0)
     assert np.array_equal(C, want)
 
