@@ -186,21 +186,31 @@ def other_configs(fastmm, torch, dev):
             probs.append({"A": rnd(m, k, torch.float32), "B": rnd(k, n, torch.float32),
                           "C": fastmm.empty(m, n, torch.float32, dev)})
             fl += 2.0 * m * k * n
-        b = fastmm.Batch(probs, graph=True)
-        v, ms = _rate(b.run, fl, sync)
-        chosen = {}
-        for i in range(len(probs)):
-            a, st = b.choice(i)
-            chosen["%s/%d" % (a, st)] = chosen.get("%s/%d" % (a, st), 0) + 1
+        def batch_run(mode):
+            b = fastmm.Batch(probs, mode=mode, graph=True)
+            v, ms = _rate(b.run, fl, sync)
+            chosen = {}
+            for i in range(len(probs)):
+                a, st = b.choice(i)
+                chosen["%s/%d" % (a, st)] = chosen.get("%s/%d" % (a, st), 0) + 1
+            errs = []
+            for p in probs[:8]:
+                D = torch.matmul(p["B"].double().t(), p["A"].double().t()).t()
+                errs.append(_err(p["C"], D, p["A"].double(), p["B"].double())[0])
+            del b
+            return v, ms, chosen, max(errs)
+
+        # FP32-accurate leaves from split-bf16 slice products (6 bf16 MFMA
+        # products per leaf, FP32 accumulation)
+        vs, mss, chosen_s, err_s = batch_run("split_bf16")
+        v, ms, chosen, err_p = batch_run("plain")
         # classical reference: one torch matmul per problem
         vt, _ = _rate(lambda: [torch.matmul(p["B"].t(), p["A"].t()) for p in probs], fl, sync)
-        errs = []
-        for p in probs[:8]:
-            D = torch.matmul(p["B"].double().t(), p["A"].double().t()).t()
-            errs.append(_err(p["C"], D, p["A"].double(), p["B"].double())[0])
         return {"workload": "256 FP32 problems, M,K,N uniform on {512,...,4096}/64, autotuned, hipGraph",
                 "total_gflop": fl / 1e9, "tflops": v, "ms": ms, "choices": chosen,
-                "torch_per_problem_matmul_tflops": vt, "normwise_err_max_first8": max(errs)}
+                "torch_per_problem_matmul_tflops": vt, "normwise_err_max_first8": err_p,
+                "split_bf16": {"tflops": vs, "ms": mss, "choices": chosen_s,
+                               "normwise_err_max_first8": err_s}}
 
     for name, fn in (("c1", c1), ("c2ts", c2ts), ("c3", c3), ("c4", c4), ("c5", c5)):
         guarded(name, fn)

@@ -43,7 +43,7 @@ struct CostModel {
 };
 
 CostModel model_for(fm_dtype dt, fm_mode mode) {
-  if (mode == FM_MODE_SPLIT_BF16) return {150.0, 5.0, 4.0};
+  if (mode == FM_MODE_SPLIT_BF16) return {190.0, 5.0, 4.0};
   if (mode == FM_MODE_OZAKI_INT8) return {60.0, 5.0, 4.0};
   return dt == FM_F64 ? CostModel{60.0, 5.0, 4.0} : CostModel{100.0, 5.0, 4.0};
 }
