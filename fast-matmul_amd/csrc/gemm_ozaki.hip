@@ -115,9 +115,10 @@ __global__ void __launch_bounds__(256) fm_ozaki_prep_rows(FmOzakiArgs args) {
 // ---- B side: one work group per 64 columns of one operand ------------------
 // Column exponents to ws + b_exp, digits to ws + b_slc.  The exponent of a
 // column is a reduction along its K contiguous entries (one wave per column,
-// coalesced); the digits are cut with the 64 lanes of a wave on 64 columns,
-// each lane 16 consecutive k of its column (one 128-byte line), so the digit
-// groups go out as consecutive 16-byte slots.
+// coalesced).  Each 64-deep K chunk of the 64 columns is then loaded along
+// K (one wave per column, 512-byte runs) into LDS and cut with the 64 lanes
+// of a wave on 64 columns, each lane 16 consecutive k of its column, so the
+// digit groups go out as consecutive 16-byte slots.
 template <int S>
 __global__ void __launch_bounds__(256) fm_ozaki_prep_cols(FmOzakiArgs args) {
   const FmOzakiDesc d = args.descs[blockIdx.y];
@@ -146,18 +147,22 @@ __global__ void __launch_bounds__(256) fm_ozaki_prep_cols(FmOzakiArgs args) {
       if (col < d.N) ex[col] = e;
     }
   }
-  __syncthreads();
+  __shared__ double tile[64][65];  // [column][k], padded
+  __syncthreads();                 // exponents published
   const int cl = tid & 63, kq = tid >> 6, col = c0 + cl;
-  const bool cok = col < d.N;
   const int e = es[cl];
-  const double* bc = B + (int64_t)col * ldb;
   for (int k0 = 0; k0 < d.Kp; k0 += 64) {
+    if (k0 > 0) __syncthreads();  // previous chunk consumed
+    for (int q = 0; q < 16; ++q) {
+      const int c2 = wave * 16 + q, k = k0 + lane;
+      tile[c2][lane] = (c0 + c2 < d.N && k < d.K) ? B[(int64_t)(c0 + c2) * ldb + k] : 0.0;
+    }
+    __syncthreads();
     const int kb = k0 + 16 * kq;
     int8_t dig[S][16];
 #pragma unroll
     for (int q = 0; q < 16; ++q) {
-      const int k = kb + q;
-      const double v = (cok && k < d.K) ? ldexp(bc[k], -e) : 0.0;
+      const double v = ldexp(tile[cl][16 * kq + q], -e);
       int8_t t[S];
       fm_digits<S>(v, t);
 #pragma unroll

@@ -238,10 +238,10 @@ hipError_t fm_gemm_launch_bf16x3(int cfg, const FmGemmArgs& a, hipStream_t s) {
 
 // ---- slice planes prepared once per operand -------------------------------------
 // One work group cuts 64 rows x one 32-deep K slice of one operand: each
-// thread 8 consecutive k of one row.  Sources with rows contiguous (A) put
-// the 64 lanes of a wave on 64 rows (coalesced loads along the column);
-// sources with K contiguous (B) give each row 4 lanes (32-byte runs).  Each
-// thread writes its 8 k of the three planes (3 x 16 bytes).
+// thread 8 consecutive k of one row, the 4 threads of a row in adjacent
+// lanes, so a wave writes 16 whole 192-byte row chunks.  Sources with rows
+// contiguous (A) load 64-byte runs down each column; sources with K
+// contiguous (B) 128-byte runs along each row.
 __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first) {
   const FmSplitPrep d = args.preps[first + blockIdx.z];
   const int r0 = blockIdx.x * 64, kc = blockIdx.y;
@@ -250,8 +250,7 @@ __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first
       reinterpret_cast<const float*>(fm_base(args.bases, d.src.role)) + d.src.off;
   char* ws = reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]);
   const int tid = threadIdx.x;
-  const int r = d.kcontig ? tid >> 2 : tid & 63;
-  const int q = d.kcontig ? tid & 3 : tid >> 6;
+  const int r = tid >> 2, q = tid & 3;
   const int row = r0 + r, k = kc * 32 + 8 * q;
   const bool rok = row < d.rows;
   const int64_t ld = d.src.ld;
