@@ -64,7 +64,9 @@ enum FmTileCfg : int {
   FM_TILE_SP_256x128 = 25,     // 4x2 waves, LDS double-buffered
   FM_TILE_SP_128x256 = 26,     // 2x4 waves, LDS double-buffered
   FM_TILE_SP_128x128_DB = 27,  // 2x2 waves, LDS double-buffered
-  FM_TILE_NUM = 28
+  FM_TILE_SP_256x128_IL = 28,  // as 25, fragment reads interleaved with the MFMAs
+  FM_TILE_SP_128x256_IL = 29,  // as 26, fragment reads interleaved with the MFMAs
+  FM_TILE_NUM = 30
 };
 
 // configurations that run on prepared slice planes
@@ -86,7 +88,8 @@ inline void fm_tile_dims(int cfg, int* bm, int* bn) {
                                         {128, 64},  {64, 128},  {128, 128}, {256, 128},
                                         {128, 128}, {128, 256}, {256, 128}, {128, 256},
                                         {128, 128}, {128, 128}, {128, 128}, {128, 256},
-                                        {128, 128}, {256, 128}, {128, 256}, {128, 128}};
+                                        {128, 128}, {256, 128}, {128, 256}, {128, 128},
+                                        {256, 128}, {128, 256}};
   *bm = d[cfg][0];
   *bn = d[cfg][1];
 }

@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first
 // fm_gemm_bf16x3_kernel, so both kernels give the same bits.
 __device__ __forceinline__ int fm_sp_swz(int r) { return (r >> 1) & 3; }
 
-template <int BM, int BN, int WM, int WN, bool DB>
+template <int BM, int BN, int WM, int WN, bool DB, bool IL = false>
 __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) {
   constexpr int NT = 64 * WM * WN;
   constexpr int WTM = BM / WM, WTN = BN / WN;
@@ -380,18 +380,26 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
   for (int kt = 0; kt < nkc; ++kt) {
     const int buf = DB ? kt & 1 : 0;
     if (kt + 1 < nkc) gload(kt + 1);
+    // B fragments and the first A row first; each further A row is read
+    // while the previous row's MFMAs run (pinned with scheduling-group
+    // barriers when IL: 15 reads, then per row 3 x (8 MFMAs, 1 read))
     fm_bf16x8 af[3][TM], bf[3][TN];
 #pragma unroll
-    for (int p = 0; p < 3; ++p) {
-#pragma unroll
-      for (int i = 0; i < TM; ++i)
-        af[p][i] = *reinterpret_cast<const fm_bf16x8*>(&As[buf][p][wm * WTM + i * 16 + fr][fsw]);
+    for (int p = 0; p < 3; ++p)
 #pragma unroll
       for (int j = 0; j < TN; ++j)
         bf[p][j] = *reinterpret_cast<const fm_bf16x8*>(&Bs[buf][p][wn * WTN + j * 16 + fr][fsw]);
-    }
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
+    for (int p = 0; p < 3; ++p)
+      af[p][0] = *reinterpret_cast<const fm_bf16x8*>(&As[buf][p][wm * WTM + fr][fsw]);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+      if (i + 1 < TM) {
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+          af[p][i + 1] =
+              *reinterpret_cast<const fm_bf16x8*>(&As[buf][p][wm * WTM + (i + 1) * 16 + fr][fsw]);
+      }
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
         fm_f32x4 c = acc[i][j];
@@ -403,6 +411,19 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
         c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[0][i], c, 0, 0, 0);
         acc[i][j] = c;
       }
+    }
+    if constexpr (IL) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 3 * TN + 3, 0);  // B fragments, A row 0
+#pragma unroll
+      for (int i = 0; i + 1 < TM; ++i)
+#pragma unroll
+        for (int p = 0; p < 3; ++p) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 2 * TN, 0);  // a third of row i's MFMAs
+          __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);       // one read of row i + 1
+        }
+      __builtin_amdgcn_sched_group_barrier(0x008, 6 * TN, 0);      // the last row
+      __builtin_amdgcn_sched_barrier(0);
+    }
     if (DB) {
       if (kt + 1 < nkc) sstore(buf ^ 1);
       __syncthreads();
@@ -454,6 +475,12 @@ hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, hipStream_t s) {
       break;
     case FM_TILE_SP_128x256:
       hipLaunchKernelGGL((fm_split_gemm<128, 256, 2, 4, true>), dim3(grid), dim3(512), 0, s, a);
+      break;
+    case FM_TILE_SP_256x128_IL:
+      hipLaunchKernelGGL((fm_split_gemm<256, 128, 4, 2, true, true>), dim3(grid), dim3(512), 0, s, a);
+      break;
+    case FM_TILE_SP_128x256_IL:
+      hipLaunchKernelGGL((fm_split_gemm<128, 256, 2, 4, true, true>), dim3(grid), dim3(512), 0, s, a);
       break;
     case FM_TILE_SP_128x128_DB:
       hipLaunchKernelGGL((fm_split_gemm<128, 128, 2, 2, true>), dim3(grid), dim3(256), 0, s, a);

@@ -345,8 +345,8 @@ const TileCand kF32Tiles[] = {
     {FM_TILE_64x64, FM_TILE_64x64, 0.66, 4}};
 // split-bf16: kernels on prepared slice planes (any shape: the planes are
 // padded to whole tiles)
-const TileCand kSplitTiles[] = {{FM_TILE_SP_256x128, FM_TILE_SP_256x128, 1.00, 1},
-                                {FM_TILE_SP_128x256, FM_TILE_SP_128x256, 1.00, 1},
+const TileCand kSplitTiles[] = {{FM_TILE_SP_256x128_IL, FM_TILE_SP_256x128_IL, 1.00, 1},
+                                {FM_TILE_SP_128x256_IL, FM_TILE_SP_128x256_IL, 1.00, 1},
                                 {FM_TILE_SP_128x128, FM_TILE_SP_128x128, 0.95, 2},
                                 {FM_TILE_SP_128x128_DB, FM_TILE_SP_128x128_DB, 0.90, 1}};
 // edge kernels run slower than whole-tile ones (predicated loads)

@@ -18,6 +18,9 @@ reference's std::chrono timing around its recursion
 (algorithms/strassen.hpp:708-721).
 
     python -m fastmm.benchmark --alg FAST424_26_206 --steps 1,2 --sizes 4096,8192
+
+``--mode split_bf16`` (FP32 data) or ``--mode ozaki_int8`` (FP64 data) runs
+the multiplies with those leaf products (fm_fastmatmul's mode argument).
 """
 from __future__ import annotations
 
@@ -27,7 +30,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 
 import torch
 
-from . import Matrix, RunAlgorithm, tables
+from . import Matrix, RunAlgorithm, fastmatmul, tables
 
 
 def enum_name(algorithm: str) -> str:
@@ -42,7 +45,8 @@ def enum_name(algorithm: str) -> str:
 
 
 def single_benchmark(algorithm: str, m: int, k: int, n: int, num_steps: int,
-                     trials: int = 5, dtype=torch.float64, seed: int = 0) -> float:
+                     trials: int = 5, dtype=torch.float64, seed: int = 0,
+                     mode: str = "plain") -> float:
     """Median device ms of `trials` multiplies (benchmarks/matmul_benchmarks.cpp:23-41)."""
     g = torch.Generator(device="cuda")
     g.manual_seed(seed)
@@ -51,7 +55,11 @@ def single_benchmark(algorithm: str, m: int, k: int, n: int, num_steps: int,
         A = Matrix(torch.rand((k, m), generator=g, device="cuda", dtype=dtype).t())
         B = Matrix(torch.rand((n, k), generator=g, device="cuda", dtype=dtype).t())
         C = Matrix(m, n, dtype=dtype)
-        if algorithm in tables.ENUM_TO_NAMESPACE:
+        if mode != "plain":
+            ns = tables.ENUM_TO_NAMESPACE.get(algorithm, algorithm)
+            x = tables.default_x(algorithm, num_steps) if algorithm in tables.ENUM_TO_NAMESPACE else 1e-8
+            _, ms = fastmatmul(ns, A.data, B.data, C.data, num_steps, x, mode=mode, timed=True)
+        elif algorithm in tables.ENUM_TO_NAMESPACE:
             ms = RunAlgorithm(algorithm, A, B, C, num_steps)
         else:
             from . import alg
@@ -62,13 +70,14 @@ def single_benchmark(algorithm: str, m: int, k: int, n: int, num_steps: int,
 
 
 def benchmark_set(algorithm: str, shapes: Sequence[Tuple[int, int, int]],
-                  steps: Sequence[int], trials: int = 5, dtype=torch.float64) -> List[str]:
+                  steps: Sequence[int], trials: int = 5, dtype=torch.float64,
+                  mode: str = "plain") -> List[str]:
     """One output line per depth (benchmarks/matmul_benchmarks.cpp:45-60)."""
     lines = []
     for s in steps:
         pts = []
         for (m, k, n) in shapes:
-            ms = single_benchmark(algorithm, m, k, n, s, trials, dtype)
+            ms = single_benchmark(algorithm, m, k, n, s, trials, dtype, mode=mode)
             pts.append(" %d %d %d %d %s ; " % (m, k, n, s, _fmt(ms)))
         lines.append("%s_%d = [%s];" % (enum_name(algorithm), s, "".join(pts)))
     return lines
@@ -115,6 +124,8 @@ def main(argv: Optional[Sequence[str]] = None):
                     help="comma-separated square sizes, or PxQxR shapes")
     ap.add_argument("--trials", type=int, default=5)
     ap.add_argument("--fp32", action="store_true")
+    ap.add_argument("--mode", default="plain", choices=["plain", "split_bf16", "ozaki_int8"],
+                    help="leaf products (split_bf16 needs --fp32, ozaki_int8 FP64 data)")
     a = ap.parse_args(argv)
     shapes = []
     for tok in a.sizes.split(","):
@@ -122,7 +133,7 @@ def main(argv: Optional[Sequence[str]] = None):
         shapes.append(tuple(dims) if len(dims) == 3 else (dims[0],) * 3)
     steps = [int(v) for v in a.steps.split(",")]
     dtype = torch.float32 if a.fp32 else torch.float64
-    for line in benchmark_set(a.alg, shapes, steps, a.trials, dtype):
+    for line in benchmark_set(a.alg, shapes, steps, a.trials, dtype, mode=a.mode):
         print(line, flush=True)
 
 

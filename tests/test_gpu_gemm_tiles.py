@@ -12,7 +12,7 @@ from cases import make_inputs
 pytestmark = pytest.mark.gpu
 
 DEV = "cuda:0"
-N_CFG = 24  # plain FP64 / FP32 configurations (24-27 are split_bf16 only)
+N_CFG = 24  # plain FP64 / FP32 configurations (24-29 are split_bf16 only)
 
 
 def _dev(a, fm):

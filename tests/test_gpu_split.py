@@ -100,9 +100,9 @@ def test_split_accuracy_fp32_level(fm, oracle_mod, dist, alg, steps):
     assert nw <= 4 * nwp + 2.0 ** -24
 
 
-# FmTileCfg: 0 = in-loop split kernel (128 x 128), 24-27 = kernels on
-# prepared slice planes
-INLOOP, PLANE_CFGS = 0, (24, 25, 26, 27)
+# FmTileCfg: 0 = in-loop split kernel (128 x 128), 24-29 = kernels on
+# prepared slice planes (28, 29: fragment reads interleaved with the MFMAs)
+INLOOP, PLANE_CFGS = 0, (24, 25, 26, 27, 28, 29)
 
 
 @pytest.fixture
