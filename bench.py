@@ -363,6 +363,13 @@ def main():
         "cpu_baseline": cpu,
         "configs": configs,
     }
+    # whole multiply against its roofline: leaf flops at the FP64 MFMA peak
+    # plus the stages' algorithmic bytes at the HBM peak (stages run one
+    # after another, so the bounds add)
+    b_add = sum(stages[k].get("algorithmic_GB", 0.0) for k in ("S", "T", "W")) * 1e9
+    t_roof = F_leaf / (PEAK_FP64_TF * 1e12) + b_add / (PEAK_HBM_GBS * 1e9)
+    out["multiply_roofline"] = {"t_roof_ms": t_roof * 1e3, "t_measured_ms": t * 1e3,
+                                "frac": t_roof / t, "leaf_flops": F_leaf, "stage_bytes": b_add}
     print(json.dumps(out))
 
 
