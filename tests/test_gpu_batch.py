@@ -82,3 +82,40 @@ def test_batch_autotuner_choices_and_fp32_accuracy(fm, oracle_mod):
         got = p["C"].cpu().numpy()[rows, cols].astype(np.float64)
         assert np.all(np.abs(got - q) <= 64 * (1 + st) * 2.0 ** -24 * absab), (alg, st)
 
+
+
+def test_batch_split_planes_match_single_calls(fm):
+    """split_bf16 batches: one preparation launch for every problem's
+    operands, one persistent plane launch.  The plane kernels compute the
+    same slice products in the same order whatever the tile, so every
+    problem equals its own single call bit for bit (explicit algorithms with
+    peeling, classical, alpha/beta), also through graph replays."""
+    shapes = [(576, 512, 640), (700, 1000, 515), (512, 576, 960), (1088, 520, 512),
+              (640, 640, 640)]
+    probs, host = _problems(fm, shapes, torch.float32, "u11", 5)
+    probs[0].update(alg="strassen", num_steps=2)
+    probs[1].update(alg="grey423_20_144", num_steps=1, alpha=2.0, beta=-1.0)
+    probs[2].update(alg="smirnov333_23_139", num_steps=1)
+    probs[3].update(alg=None, num_steps=0, beta=0.5)
+    probs[4].update(alg="strassen", num_steps=1)
+    C0s = [p["C"].clone() for p in probs]
+    want = []
+    for p, C0 in zip(probs, C0s):
+        C = C0.clone()
+        alpha, beta = p.get("alpha", 1.0), p.get("beta", 0.0)
+        if p.get("alg"):
+            fm.fastmatmul(p["alg"], p["A"], p["B"], C, p["num_steps"], alpha=alpha, beta=beta,
+                          mode="split_bf16")
+        else:  # a batch's classical problem honours beta, as gemm does
+            fm.gemm(p["A"], p["B"], C, alpha, beta, mode="split_bf16")
+        want.append(C.cpu().numpy())
+    b = fm.Batch(probs, mode="split_bf16", graph=True)
+    for rep in range(2):
+        for p, C0 in zip(probs, C0s):
+            p["C"].copy_(C0)
+        torch.cuda.synchronize()
+        b.run()
+        torch.cuda.synchronize()
+        for i, (p, w) in enumerate(zip(probs, want)):
+            got = p["C"].cpu().numpy()
+            assert np.array_equal(got.view(np.uint32), w.view(np.uint32)), (rep, i, b.choice(i))
