@@ -426,6 +426,50 @@ int best_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods, d
 }
 }  // namespace
 
+namespace {
+// Whole-tile carve-out: a launch whose products are not all whole tiles runs
+// every tile on the predicated edge kernel.  Cutting each ragged product
+// into its whole-tile core (the unpredicated kernel) and the row / column
+// strips left over (a second launch, its own tile) is worth it when the
+// strips are a small share; every output element is still computed by the
+// same K-ordered MFMA sequence.  Returns false (prods unchanged) when it
+// does not pay or does not apply.
+FmGemmDesc gsub(const FmGemmDesc& d, int64_t i0, int64_t j0, int64_t m, int64_t n) {
+  return gdesc(op_sub(d.a, i0, 0), op_sub(d.b, 0, j0), op_sub(d.c, i0, j0), m, n, d.K, d.alpha,
+               d.beta);
+}
+
+bool carve_full(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods, std::vector<FmGemmDesc>& strips,
+                int* cfg_core, int* cfg_strips) {
+  if (g_tile_override >= 0 || P.mode != FM_MODE_PLAIN) return false;
+  const TileCand* c = find_cand(P.dtype, P.mode, cfg);
+  if (!c || whole_tiles(P.dtype, c->cfg_full, prods)) return false;
+  int bm, bn;
+  fm_tile_dims(c->cfg_full, &bm, &bn);
+  const int bk = fm_tile_bk(c->cfg_full), vl = P.dtype == FM_F64 ? 2 : 4;
+  std::vector<FmGemmDesc> core, st;
+  for (const FmGemmDesc& d : prods) {
+    if (d.K % bk || d.K == 0 || d.a.off % vl || d.a.ld % vl || d.b.off % vl || d.b.ld % vl)
+      return false;
+    const int64_t Mf = d.M / bm * bm, Nf = d.N / bn * bn;
+    if (Mf > 0 && Nf > 0) core.push_back(gsub(d, 0, 0, Mf, Nf));
+    if (d.N > Nf) st.push_back(gsub(d, 0, Nf, d.M, d.N - Nf));          // right strip, all rows
+    if (d.M > Mf && Nf > 0) st.push_back(gsub(d, Mf, 0, d.M - Mf, Nf));  // bottom strip
+  }
+  if (core.empty() || st.empty()) return false;
+  double t_strips = 0;
+  const int cs = best_tile(P.dtype, P.mode, st, &t_strips);
+  const double t_core = tile_time(*c, count_tiles(c->cfg_full, core), true);
+  const double t_all = tile_time(*c, count_tiles(cfg, prods), false);
+  if (t_core + t_strips >= 0.97 * t_all) return false;
+  prods = std::move(core);
+  strips = std::move(st);
+  *cfg_core = c->cfg_full;
+  *cfg_strips = cs;
+  return true;
+}
+}  // namespace
+
 int g_split_remainder = 1;
 
 int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods) {
@@ -719,10 +763,13 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
       const int cfg = pick_tile(P.dtype, P.mode, leaves);
       if (fm_tile_planes(cfg) && P.mode != FM_MODE_SPLIT_BF16)
         return set_err(FM_ERR_INVALID_ARG, "tile config " + std::to_string(cfg) + " is for split_bf16 leaves");
-      std::vector<FmGemmDesc> rest;
-      int cfg_rest = cfg, cfg_head = cfg;
+      std::vector<FmGemmDesc> rest, strips;
+      int cfg_rest = cfg, cfg_head = cfg, cfg_main = cfg, cfg_strips = cfg;
+      if (persistent_grid == 0 && g_split_remainder &&
+          carve_full(P, cfg, leaves, strips, &cfg_main, &cfg_strips))
+        cfg_rest = cfg_head = cfg_main;
       if (persistent_grid == 0 && g_split_remainder)
-        split_remainder(P, cfg, leaves, rest, &cfg_rest, &cfg_head);
+        split_remainder(P, cfg_main, leaves, rest, &cfg_rest, &cfg_head);
       auto leaf = [&](std::vector<FmGemmDesc> prods, int c) {
         if (fm_tile_planes(c)) return split_launch(std::move(prods), c, &wsb);
         return gemm_launch(std::move(prods), c, FM_STAGE_LEAF, P.dtype);
@@ -731,6 +778,7 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
       P.launches.back().gm.grid = persistent_grid;
       P.launches.back().sp.grid = persistent_grid;
       if (!rest.empty()) P.launches.push_back(leaf(std::move(rest), cfg_rest));
+      if (!strips.empty()) P.launches.push_back(leaf(std::move(strips), cfg_strips));
     }
   }
   int maxd = 0;
