@@ -771,6 +771,11 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
       if (persistent_grid == 0 && g_split_remainder)
         split_remainder(P, cfg_main, leaves, rest, &cfg_rest, &cfg_head);
       auto leaf = [&](std::vector<FmGemmDesc> prods, int c) {
+        int64_t kmax = 0;
+        for (const FmGemmDesc& d : prods) kmax = std::max<int64_t>(kmax, d.K);
+        // the preparation grid has one row of work groups per 32-deep K
+        // slice (at most 65535): deeper leaves split on the fly instead
+        if (fm_tile_planes(c) && kmax > 65535LL * 32) c = FM_TILE_128x128;
         if (fm_tile_planes(c)) return split_launch(std::move(prods), c, &wsb);
         return gemm_launch(std::move(prods), c, FM_STAGE_LEAF, P.dtype);
       };

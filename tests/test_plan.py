@@ -102,6 +102,14 @@ def test_split_leaves_on_slice_planes():
     assert p["workspace_bytes"] == 384 * 128 * 6 + 256 * 128 * 6
 
 
+def test_split_leaves_deeper_than_the_prep_grid_split_on_the_fly():
+    # more than 65535 K slices of 32: the in-loop split kernel (FmTileCfg 0)
+    p = fastmm.plan_info("strassen", 64, 2200000, 64, 0, dtype=torch.float32, mode="split_bf16")
+    assert p["leaf"][0]["cfg"] == 0
+    p = fastmm.plan_info("strassen", 64, 2000000, 64, 0, dtype=torch.float32, mode="split_bf16")
+    assert p["leaf"][0]["cfg"] >= 24
+
+
 def test_plane_tiles_need_split_mode():
     fastmm.set_gemm_tile(25)
     try:
