@@ -36,7 +36,9 @@ struct Arena {
   void* p = nullptr;
   size_t bytes = 0;
 };
-std::map<int, Arena> g_arenas;
+// one cached workspace per (device, stream): calls on one stream are
+// ordered by it, calls on different streams never share a workspace
+std::map<std::pair<int, hipStream_t>, Arena> g_arenas;
 
 std::string plan_key(int dev, const char* alg, int dt, int mode, int64_t m, int64_t k, int64_t n,
                      int64_t lda, int64_t ldb, int64_t ldc, int steps, double x, double alpha,
@@ -141,11 +143,11 @@ fm_status get_plan(const char* alg_name, const FmAlgDesc* alg, fm_dtype dt, fm_m
   return FM_OK;
 }
 
-fm_status arena_get(size_t bytes, void** p) {
+fm_status arena_get(size_t bytes, hipStream_t stream, void** p) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return hip_err(e, "hipGetDevice");
-  Arena& a = g_arenas[dev];
+  Arena& a = g_arenas[std::make_pair(dev, stream)];
   if (a.bytes < bytes) {
     if (a.p) {
       e = hipDeviceSynchronize();
@@ -270,7 +272,8 @@ double launch_bytes(const Launch& L, double es) {
   return 0.0;
 }
 
-fm_status plan_workspace(const Plan& P, void* workspace, size_t workspace_bytes, void** ws) {
+fm_status plan_workspace(const Plan& P, void* workspace, size_t workspace_bytes, hipStream_t stream,
+                         void** ws) {
   *ws = workspace;
   if (P.ws_bytes == 0) return FM_OK;
   if (workspace) {
@@ -278,7 +281,7 @@ fm_status plan_workspace(const Plan& P, void* workspace, size_t workspace_bytes,
       return set_err(FM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(P.ws_bytes) + " bytes");
     return FM_OK;
   }
-  return arena_get(P.ws_bytes, ws);
+  return arena_get(P.ws_bytes, stream, ws);
 }
 
 }  // namespace
@@ -306,7 +309,7 @@ fm_status fm_fastmatmul(const char* alg_name, fm_dtype dtype, fm_mode mode, int 
   st = get_plan(alg_name, alg, dtype, mode, m, k, n, lda, ldb, ldc, num_steps, x, alpha, beta, &P);
   if (st != FM_OK) return st;
   void* ws = nullptr;
-  st = plan_workspace(*P, workspace, workspace_bytes, &ws);
+  st = plan_workspace(*P, workspace, workspace_bytes, (hipStream_t)stream, &ws);
   if (st != FM_OK) return st;
   return timed_run(*P, A, B, C, ws, (hipStream_t)stream, ms_out);
 }
@@ -365,7 +368,7 @@ fm_status fm_fastmatmul_scaled(const char* alg_name, fm_dtype dtype, fm_mode mod
     if (workspace_bytes < off)
       return set_err(FM_ERR_WORKSPACE, "workspace too small: need " + std::to_string(off) + " bytes");
   } else {
-    st = arena_get(off, &ws);
+    st = arena_get(off, (hipStream_t)stream, &ws);
     if (st != FM_OK) return st;
   }
   char* w = (char*)ws;
@@ -472,7 +475,7 @@ fm_status fm_gemm(fm_dtype dtype, fm_mode mode, int m, int k, int n, const void*
   st = get_plan(nullptr, nullptr, dtype, mode, m, k, n, lda, ldb, ldc, 0, 0.0, alpha, beta, &P);
   if (st != FM_OK) return st;
   void* ws = nullptr;
-  st = plan_workspace(*P, nullptr, 0, &ws);
+  st = plan_workspace(*P, nullptr, 0, (hipStream_t)stream, &ws);
   if (st != FM_OK) return st;
   return P->run(A, B, C, ws, (hipStream_t)stream);
 }

@@ -62,8 +62,9 @@ typedef enum { FM_MODE_PLAIN = 0, FM_MODE_SPLIT_BF16 = 1, FM_MODE_OZAKI_INT8 = 2
  * reference's namespace name ("strassen", "grey424_26_206", ...).  `x` is
  * the APA parameter (ignored by exact algorithms).  `stream` is a
  * hipStream_t (NULL = default stream).  `workspace` may be NULL (the library
- * then uses its own cached device arena) or a device buffer of at least
- * fm_workspace_query() bytes.  If `ms_out` is non-NULL the call waits for
+ * then uses its own cached device arena, one per device and stream, so
+ * calls on different streams may run concurrently) or a device buffer of at
+ * least fm_workspace_query() bytes.  If `ms_out` is non-NULL the call waits for
  * completion and stores the elapsed device time in ms (the reference's
  * FastMatmul returns its elapsed ms, algorithms/strassen.hpp:708-721). */
 fm_status fm_fastmatmul(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k, int n,

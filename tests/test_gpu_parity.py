@@ -231,3 +231,25 @@ def test_cpp_header_program(tmp_path):
                        timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout
+
+
+def test_concurrent_streams_use_private_workspaces(fm):
+    """Calls without a caller workspace on two streams at once: each
+    (device, stream) has its own cached workspace, so interleaved calls give
+    the sequential results bit for bit."""
+    n = 2048
+    A1, B1, _ = make_inputs(n, n, n, 61, "u11")
+    A2, B2, _ = make_inputs(n, n, n, 62, "u11")
+    A1, B1, A2, B2 = (_to_dev(x, fm) for x in (A1, B1, A2, B2))
+    ref1 = fm.fastmatmul("grey424_26_206", A1, B1, None, 2)
+    ref2 = fm.fastmatmul("grey424_26_206", A2, B2, None, 2)
+    C1, C2 = fm.empty(n, n), fm.empty(n, n)
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for _ in range(6):
+        with torch.cuda.stream(s1):
+            fm.fastmatmul("grey424_26_206", A1, B1, C1, 2)
+        with torch.cuda.stream(s2):
+            fm.fastmatmul("grey424_26_206", A2, B2, C2, 2)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, ref1) and torch.equal(C2, ref2)
