@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -31,7 +32,12 @@ const FmAlgDesc* fm_find_alg(const char* name) {
 
 namespace {
 
+// Plan cache, bounded: the key holds the shape, leading dimensions, alpha,
+// beta and x, so a caller that varies alpha from call to call would
+// otherwise grow it without end.  The oldest plan goes first.
+constexpr size_t kMaxPlans = 256;
 std::map<std::string, std::unique_ptr<Plan>> g_plans;
+std::deque<std::string> g_plan_order;
 struct Arena {
   void* p = nullptr;
   size_t bytes = 0;
@@ -138,8 +144,18 @@ fm_status get_plan(const char* alg_name, const FmAlgDesc* alg, fm_dtype dt, fm_m
   if (st != FM_OK) return st;
   st = P->upload();
   if (st != FM_OK) return st;
+  if (g_plans.size() >= kMaxPlans) {
+    // kernels already queued may still read the evicted plan's descriptors
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_err(e, "hipDeviceSynchronize");
+    while (g_plans.size() >= kMaxPlans && !g_plan_order.empty()) {
+      g_plans.erase(g_plan_order.front());
+      g_plan_order.pop_front();
+    }
+  }
   *out = P.get();
   g_plans[key] = std::move(P);
+  g_plan_order.push_back(key);
   return FM_OK;
 }
 
@@ -524,6 +540,7 @@ fm_status fm_set_gemm_tile(int cfg) {
   if (cfg < -1 || cfg >= FM_TILE_NUM) return set_err(FM_ERR_INVALID_ARG, "unknown tile config");
   g_tile_override = cfg;
   g_plans.clear();  // plans embed the tile choice
+  g_plan_order.clear();
   return FM_OK;
 }
 
@@ -555,6 +572,7 @@ void fm_release_caches(void) {
   std::lock_guard<std::mutex> lock(fm_api_mutex());
   (void)hipDeviceSynchronize();
   g_plans.clear();
+  g_plan_order.clear();
   for (auto& kv : g_arenas)
     if (kv.second.p) (void)hipFree(kv.second.p);
   g_arenas.clear();

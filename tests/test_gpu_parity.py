@@ -253,3 +253,16 @@ def test_concurrent_streams_use_private_workspaces(fm):
             fm.fastmatmul("grey424_26_206", A2, B2, C2, 2)
     torch.cuda.synchronize()
     assert torch.equal(C1, ref1) and torch.equal(C2, ref2)
+
+
+def test_plan_cache_eviction(fm):
+    """alpha is part of a plan, so 300 calls with distinct alphas exceed the
+    256-plan cache; evicted plans are rebuilt and every result stays exact."""
+    A, B, _ = make_inputs(96, 64, 80, 63, "int")
+    Ad, Bd = _to_dev(A, fm), _to_dev(B, fm)
+    exact = A @ B
+    for i in range(300):
+        C = fm.fastmatmul("strassen", Ad, Bd, None, 1, alpha=float(i + 1)).cpu().numpy()
+        assert np.array_equal(C, (i + 1) * exact), i
+    C = fm.fastmatmul("strassen", Ad, Bd, None, 1, alpha=1.0).cpu().numpy()
+    assert np.array_equal(C, exact)
