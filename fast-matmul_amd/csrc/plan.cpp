@@ -747,7 +747,32 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
   return FM_OK;
 }
 
+namespace {
+// The stage kernels walk a block's elements with 32-bit counters in a
+// grid-stride loop; blocks of 2^31 elements or more are refused (only
+// reachable with matrices near the device's memory size).
+fm_status check_stage_size(const Launch& L) {
+  if (L.kind == LK_GEMM || L.kind == LK_OZAKI || L.kind == LK_SPLIT || L.kind == LK_NONE) return FM_OK;
+  const bool two = L.kind == LK_S2 || L.kind == LK_T2 || L.kind == LK_W2;
+  const int64_t elems = two ? (int64_t)L.st.rows2 * L.st.cols2 : (int64_t)L.st.rows * L.st.cols;
+  if (elems >= (int64_t(1) << 31))
+    return set_err(FM_ERR_SHAPE, "a recursion block of " + std::to_string(elems) +
+                                     " elements exceeds the stage kernels' 2^31 limit; use more steps");
+  return FM_OK;
+}
+}  // namespace
+
 fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int persistent_grid) {
+  for (SubPlan& s : subs) {
+    for (const Launch& L : s.pre) {
+      const fm_status st = check_stage_size(L);
+      if (st != FM_OK) return st;
+    }
+    for (const Launch& L : s.w) {
+      const fm_status st = check_stage_size(L);
+      if (st != FM_OK) return st;
+    }
+  }
   for (SubPlan& s : subs)
     for (Launch& L : s.pre) P.launches.push_back(std::move(L));
   std::vector<FmGemmDesc> leaves;

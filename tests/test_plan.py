@@ -110,6 +110,14 @@ def test_split_leaves_deeper_than_the_prep_grid_split_on_the_fly():
     assert p["leaf"][0]["cfg"] >= 24
 
 
+def test_stage_blocks_beyond_32_bit_counts_are_refused():
+    # Strassen L=1 on 131072^3: 65536^2 = 2^32-element blocks; L=2 is fine
+    with pytest.raises(ValueError):
+        fastmm.plan_info("strassen", 131072, 131072, 131072, 1, dtype=torch.float32)
+    p = fastmm.plan_info("strassen", 131072, 131072, 131072, 2, dtype=torch.float32)
+    assert p["launches"]["LEAF"] >= 1
+
+
 def test_plane_tiles_need_split_mode():
     fastmm.set_gemm_tile(25)
     try:
