@@ -92,3 +92,18 @@ def test_scaling_repairs_skewed_fp32(fm, alg):
     sb = _err(fm.fastmatmul_scaled(alg, Ad, Bd, None, 2, mode="split_bf16").cpu().numpy(),
               A64, B64)
     assert sb < 1e-5, sb
+
+
+@pytest.mark.parametrize("alg", ["strassen", "smirnov333_23_139"])
+def test_scaling_with_digit_leaves(fm, alg):
+    """FP64 rows/columns scaled by 10^+-3: with the reference's outside-inside
+    scaling the int8-digit leaves reach the plain FP64 leaves' componentwise
+    error (within 4x), below the unscaled fast multiply's."""
+    A, B = _skewed(486, 486, 486, 9)
+    Ad, Bd = _dev(A, fm), _dev(B, fm)
+    unscaled = _err(fm.fastmatmul(alg, Ad, Bd, None, 2).cpu().numpy(), A, B)
+    plain = _err(fm.fastmatmul_scaled(alg, Ad, Bd, None, 2).cpu().numpy(), A, B)
+    digits = _err(fm.fastmatmul_scaled(alg, Ad, Bd, None, 2, mode="ozaki_int8").cpu().numpy(),
+                  A, B)
+    assert digits <= 4 * plain + 1e-15, (digits, plain)
+    assert digits < 0.1 * unscaled, (digits, unscaled)
