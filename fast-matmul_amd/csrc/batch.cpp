@@ -35,17 +35,23 @@ extern const FmAlgDesc* fm_find_alg(const char* name);
 namespace {
 
 // Measured on MI355X (bench.py / sweeps): leaf GEMM rate in TFLOP/s of
-// 2mnk-equivalent work, stage bandwidth, launch cost.
+// 2mnk-equivalent work, stage bandwidth, launch cost, and the cost of one
+// recursion level of a batched problem beyond its stage bytes.  On the
+// batch of 256 FP32 problems of 512-4096 (plain leaves) every problem
+// given a level cost the batch 0.1-0.3 ms more than the bytes and launches
+// predicted: its small stage and peeling launches do not fill the GPU, and
+// its odd-shaped leaves take the grouped launch's edge paths.
 struct CostModel {
   double leaf_tflops;
   double stage_tbps;
   double launch_us;
+  double level_us;
 };
 
 CostModel model_for(fm_dtype dt, fm_mode mode) {
-  if (mode == FM_MODE_SPLIT_BF16) return {190.0, 5.0, 4.0};
-  if (mode == FM_MODE_OZAKI_INT8) return {60.0, 5.0, 4.0};
-  return dt == FM_F64 ? CostModel{60.0, 5.0, 4.0} : CostModel{100.0, 5.0, 4.0};
+  if (mode == FM_MODE_SPLIT_BF16) return {190.0, 5.0, 4.0, 250.0};
+  if (mode == FM_MODE_OZAKI_INT8) return {60.0, 5.0, 4.0, 250.0};
+  return dt == FM_F64 ? CostModel{60.0, 5.0, 4.0, 250.0} : CostModel{100.0, 5.0, 4.0, 250.0};
 }
 
 // Estimated device time (us) of one multiply with `alg` at `steps` levels.
@@ -73,6 +79,7 @@ double estimate_us(fm_dtype dt, fm_mode mode, const FmAlgDesc* alg, int64_t m, i
     us += nodes * 2.0 * (ecA * (rm - erA) * (rn - ecB) + ecB * rm * rk + erA * rk * (rn - ecB)) /
           (0.5 * cm.leaf_tflops * 1e6);
     launches += 3 + (erA + ecA + ecB > 0 ? 1 : 0);
+    us += cm.level_us;
     nodes *= alg->R;
     rm = bm;
     rk = bk;
@@ -82,7 +89,14 @@ double estimate_us(fm_dtype dt, fm_mode mode, const FmAlgDesc* alg, int64_t m, i
   return us + launches * cm.launch_us;
 }
 
-const char* const kDefaultCandidates[] = {"strassen", "grey423_20_144", "smirnov333_23_139"};
+// Strassen, <4,2,3;20> in all six orientations (the reference ships each
+// permutation as its own table; for an uneven problem the one whose block
+// split matches the shape's aspect needs the fewest flops and no peeling)
+// and <3,3,3;23>
+const char* const kDefaultCandidates[] = {"strassen",       "grey423_20_144", "grey432_20_144",
+                                          "grey243_20_144", "grey234_20_144", "grey324_20_144",
+                                          "grey342_20_144", "smirnov333_23_139"};
+constexpr int kNumDefaultCandidates = (int)(sizeof(kDefaultCandidates) / sizeof(kDefaultCandidates[0]));
 
 }  // namespace
 
@@ -129,7 +143,7 @@ fm_status fm_batch_create(fm_dtype dtype, fm_mode mode, const fm_problem* proble
     return set_err(FM_ERR_UNSUPPORTED, "leaf mode not available for this dtype");
   if (!candidates || num_candidates <= 0) {
     candidates = kDefaultCandidates;
-    num_candidates = 3;
+    num_candidates = kNumDefaultCandidates;
   }
   if (max_steps < 0) max_steps = 2;
   std::vector<const FmAlgDesc*> cands;

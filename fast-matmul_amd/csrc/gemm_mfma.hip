@@ -124,11 +124,15 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
 
-  // Interior tiles of operands with 16-byte aligned columns take unpredicated
-  // 16-byte loads for every full K slice; everything else takes the
-  // predicated element loads.
+  // Interior tiles of products whose A and B columns start 16-byte aligned
+  // (checked per product: one misaligned product of a grouped launch does
+  // not slow the others; args.vec covers the base pointers) take
+  // unpredicated 16-byte loads for every full K slice; everything else
+  // takes the predicated element loads.
   static_assert(!FULL || CAN_VEC, "full-tile variant needs the vector load mapping");
-  const bool interior = FULL || (CAN_VEC && args.vec && (m0 + BM <= M) && (n0 + BN <= N));
+  const bool d_aligned = d.a.off % VL == 0 && d.a.ld % VL == 0 && d.b.off % VL == 0 && d.b.ld % VL == 0;
+  const bool interior =
+      FULL || (CAN_VEC && args.vec && d_aligned && (m0 + BM <= M) && (n0 + BN <= N));
 
   // element-load coordinates
   const int a_row = tid % BM, a_k = tid / BM;

@@ -23,7 +23,8 @@ struct FmGemmArgs {
   int32_t ndesc;
   int32_t total_tiles;
   int32_t uniform;  // all products have one shape: tile -> product by division
-  int32_t vec;      // every A/B column start is 16-byte aligned: vector loads allowed
+  int32_t vec;      // the base pointers are 16-byte aligned: vector loads allowed for the
+                    // products whose own offsets and leading dimensions keep columns aligned
   int32_t grid;     // > 0: persistent launch with this many work groups
   int32_t full;     // every product is a whole number of tiles and K slices
 };

@@ -155,9 +155,10 @@ Launch gemm_launch(std::vector<FmGemmDesc> descs, int cfg, int stage, fm_dtype d
   L.gm.ndesc = (int32_t)descs.size();
   L.gm.total_tiles = t;
   L.gm.uniform = uniform ? 1 : 0;
-  // 16-byte loads need every A/B column start aligned: even (FP64) or
-  // multiple-of-4 (FP32) offsets and leading dimensions; bases are checked
-  // at launch
+  // 16-byte loads need A/B column starts aligned: even (FP64) or
+  // multiple-of-4 (FP32) offsets and leading dimensions.  The kernels check
+  // each product's own; the base pointers are checked at launch.  The
+  // unpredicated whole-tile kernel needs every product whole and aligned.
   const int vl = dt == FM_F64 ? 2 : 4;
   const int bk = fm_tile_bk(cfg);
   bool vec = true, full = true;
@@ -165,8 +166,8 @@ Launch gemm_launch(std::vector<FmGemmDesc> descs, int cfg, int stage, fm_dtype d
     vec = vec && d.a.off % vl == 0 && d.a.ld % vl == 0 && d.b.off % vl == 0 && d.b.ld % vl == 0;
     full = full && d.M % bm == 0 && d.N % bn == 0 && d.K % bk == 0 && d.K > 0;
   }
-  L.vec_ok = vec ? 1 : 0;
-  L.gm.full = full ? 1 : 0;
+  L.vec_ok = 1;
+  L.gm.full = (full && vec) ? 1 : 0;
   L.cfg = cfg;
   L.gdescs = std::move(descs);
   return L;

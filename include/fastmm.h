@@ -89,7 +89,8 @@ fm_status fm_gemm(fm_dtype dtype, fm_mode mode, int m, int k, int n, const void*
  * problem's S/T stages, ONE grouped launch of all base-case products, then
  * the W stages and one grouped peeling launch per level.  A problem with
  * alg == NULL (or num_steps < 0) gets its algorithm (from `candidates`,
- * default {"strassen", "grey423_20_144", "smirnov333_23_139"}) and depth
+ * default "strassen", "grey423_20_144" in its six orientations
+ * (grey432/243/234/324/342_20_144) and "smirnov333_23_139") and depth
  * (0 .. max_steps, default 2) from the autotuner's cost model; depth 0 is
  * the classical product with beta honoured.  With use_graph the launch
  * sequence is captured once into a hipGraph and fm_batch_run replays it
