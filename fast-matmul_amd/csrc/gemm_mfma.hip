@@ -378,6 +378,7 @@ static hipError_t fm_gemm_launch(int cfg, const FmGemmArgs& a, hipStream_t s) {
     case FM_TILE_PP_128x128_O2: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 4, 1>(a, s); break;
     case FM_TILE_PI_128x128_O2: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 4, 2>(a, s); break;
     case FM_TILE_PI_128x256_W8: fm_gemm_go<T, 128, 256, 16, 2, 4, 1, 4, 2>(a, s); break;
+    case FM_TILE_PI_128x96_O2: fm_gemm_go<T, 128, 96, 16, 2, 2, 2, 4, 2>(a, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -67,11 +67,13 @@ enum FmTileCfg : int {
   FM_TILE_SP_128x128_DB = 27,  // 2x2 waves, LDS double-buffered
   FM_TILE_SP_256x128_IL = 28,  // as 25, fragment reads interleaved with the MFMAs
   FM_TILE_SP_128x256_IL = 29,  // as 26, fragment reads interleaved with the MFMAs
-  FM_TILE_NUM = 30
+  // plain FP64 / FP32 again: 96-column strips (e.g. 2400 = 18 x 128 + 96)
+  FM_TILE_PI_128x96_O2 = 30,   // BK 16, 2x2 waves of 64 x 48, registers for 2 waves per SIMD
+  FM_TILE_NUM = 31
 };
 
 // configurations that run on prepared slice planes
-inline bool fm_tile_planes(int cfg) { return cfg >= FM_TILE_SP_128x128 && cfg < FM_TILE_NUM; }
+inline bool fm_tile_planes(int cfg) { return cfg >= FM_TILE_SP_128x128 && cfg <= FM_TILE_SP_128x256_IL; }
 
 // K slice depth of each configuration
 inline int fm_tile_bk(int cfg) {
@@ -90,7 +92,7 @@ inline void fm_tile_dims(int cfg, int* bm, int* bn) {
                                         {128, 128}, {128, 256}, {256, 128}, {128, 256},
                                         {128, 128}, {128, 128}, {128, 128}, {128, 256},
                                         {128, 128}, {256, 128}, {128, 256}, {128, 128},
-                                        {256, 128}, {128, 256}};
+                                        {256, 128}, {128, 256}, {128, 96}};
   *bm = d[cfg][0];
   *bn = d[cfg][1];
 }

@@ -334,6 +334,7 @@ const TileCand kF64Tiles[] = {
     {FM_TILE_256x128_W8, FM_TILE_256x128_W8_U1, 0.96, 1},
     {FM_TILE_128x128_W16, FM_TILE_128x128_W16, 0.90, 1},
     {FM_TILE_128x64_W8, FM_TILE_128x64_W8, 0.88, 2},
+    {FM_TILE_PI_128x96_O2, FM_TILE_PI_128x96_O2, 0.88, 2},
     {FM_TILE_64x64, FM_TILE_64x64, 0.78, 3}};
 const TileCand kF32Tiles[] = {
     {FM_TILE_256x256_W16, -1, 1.00, 1},

@@ -12,7 +12,8 @@ from cases import make_inputs
 pytestmark = pytest.mark.gpu
 
 DEV = "cuda:0"
-N_CFG = 24  # plain FP64 / FP32 configurations (24-29 are split_bf16 only)
+# plain FP64 / FP32 configurations (24-29 are split_bf16 only)
+PLAIN_CFGS = list(range(24)) + [30]
 
 
 def _dev(a, fm):
@@ -26,7 +27,7 @@ def _reset(fm):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("cfg", range(N_CFG))
+@pytest.mark.parametrize("cfg", PLAIN_CFGS)
 def test_tile_config_exact_on_integers(fm, cfg, dtype):
     fm.set_gemm_tile(cfg)
     # whole-tile shapes (every configuration's full-tile variant) and edges
@@ -41,7 +42,7 @@ def test_tile_config_exact_on_integers(fm, cfg, dtype):
 
 @pytest.mark.parametrize("shape", [(515, 1031, 263), (512, 1024, 512)])
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 4e-16), (np.float32, 2e-7)])
-@pytest.mark.parametrize("cfg", range(N_CFG))
+@pytest.mark.parametrize("cfg", PLAIN_CFGS)
 def test_tile_config_random(fm, cfg, dtype, tol, shape):
     fm.set_gemm_tile(cfg)
     m, k, n = shape

@@ -109,14 +109,13 @@ def test_remainder_split_bit_identical(fm):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_whole_tile_carve_out_bit_identical(fm, dtype):
-    # strassen L=1 on 8200 x 1024 x 4200: 7 leaves of 4100 x 512 x 2100, each
-    # cut into a 4096 x 2048 whole-tile core and 4- and 52-wide strips
-    # (FP64; FP32 picks its own tiles), against one edge-kernel launch
-    m, k, n = 8200, 1024, 4200
+    # strassen L=1 on 8200 x 1024 x 4340: 7 leaves of 4100 x 512 x 2170, each
+    # cut into a whole-tile core and row / column strips, against one
+    # edge-kernel launch
+    m, k, n = 8200, 1024, 4340
     A, B, _ = make_inputs(m, k, n, 25, "u11", dtype)
     Ad, Bd = _dev(A, fm), _dev(B, fm)
     Cs, ps = _run(fm, "strassen", Ad, Bd, None, 1, (0, 0, True))
     Cu, pu = _run(fm, "strassen", Ad, Bd, None, 1, (0, 0, False))
-    if dtype == np.float64:
-        assert (ps["LEAF"][1], pu["LEAF"][1]) == (2, 1)
+    assert (ps["LEAF"][1], pu["LEAF"][1]) == (2, 1)
     assert torch.equal(Cs, Cu)
