@@ -23,7 +23,7 @@ import numpy as np
 TEST_SHAPES = [(257, 500, 55), (300, 500, 100), (366, 500, 200), (101, 500, 340),
                (402, 500, 562), (600, 500, 671)]
 
-# algorithms of tests/matmul_tests.cpp:148-174 plus the BASELINE configs
+# algorithms of tests/matmul_tests.cpp:148-174 plus those of the benchmark workloads
 PARITY_ALGS = ["strassen", "grey333_23_125", "grey322_11_50", "grey332_15_103",
                "grey323_15_103", "grey323_15_89", "grey333_23_152", "grey333_23_221",
                "grey432_20_144", "grey423_20_144", "grey324_20_144", "grey342_20_144",

@@ -130,7 +130,7 @@ def test_plane_tiles_need_split_mode():
 def test_stage_bytes_match_closed_forms():
     """Algorithmic bytes per stage, grey424_26_206 L=2 FP64 8192^3: one-level
     S = 8 (blocks of level 1 + 26 x blocks of level 2) x (8 reads + 22 writes),
-    the per-level figures of the survey (8.56 / 7.99 / 3.70 GB); two-level S
+    the per-level figures 8.56 / 7.99 / 3.70 GB; two-level S
     reads A once and writes 26 x (22 outputs + 4 views of 22 products)."""
     fastmm.set_stage_fusion(False)
     b = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)["stage_bytes"]
