@@ -215,9 +215,13 @@ fm_status fm_batch_create(fm_dtype dtype, fm_mode mode, const fm_problem* proble
     b->alg_names.push_back(p.alg ? p.alg->name : "classical");
     b->steps.push_back(p.alg ? steps : 0);
   }
-  // one persistent grouped launch for every problem's base-case products:
-  // 2048 work groups (8 per CU) loop over the tiles of all problems
-  fm_status st = assemble(P, subs, ws, 2048);
+  // one grouped launch for every problem's base-case products, one work
+  // group per tile: the tiles' K runs 512-4096 here, so the hardware
+  // dispatcher, which starts a tile wherever a slot frees, balances the CUs
+  // better than a persistent grid taking every grid-th tile (measured on 256
+  // problems: 107.6 against 103.4 TFLOP/s with 2048 persistent work groups).
+  // Whole-tile carving and the remainder split apply as for one multiply.
+  fm_status st = assemble(P, subs, ws, 0);
   if (st != FM_OK) return st;
   st = P.upload();
   if (st != FM_OK) return st;

@@ -97,7 +97,8 @@ fm_status build_dfs(fm_dtype dt, const Problem& p, const std::vector<Dims>& dims
 //   all S/T stages; one grouped leaf launch; per level (deepest first) the W
 //   stages, then one grouped launch of that level's peeling products.
 // persistent_grid > 0 runs the grouped leaf launch with that many work
-// groups, each looping over tiles.
+// groups, each looping over tiles (no carving or remainder split then);
+// 0 launches one work group per tile.
 fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int persistent_grid = 0);
 
 int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods);

@@ -87,7 +87,7 @@ def test_batch_autotuner_choices_and_fp32_accuracy(fm, oracle_mod):
 
 def test_batch_split_planes_match_single_calls(fm):
     """split_bf16 batches: one preparation launch for every problem's
-    operands, one persistent plane launch.  The plane kernels compute the
+    operands, one grouped plane launch.  The plane kernels compute the
     same slice products in the same order whatever the tile, so every
     problem equals its own single call bit for bit (explicit algorithms with
     peeling, classical, alpha/beta), also through graph replays."""
