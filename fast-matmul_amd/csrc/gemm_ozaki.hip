@@ -281,26 +281,33 @@ __global__ void __launch_bounds__(256) fm_ozaki_gemm(FmOzakiArgs args) {
   }
 }
 
-// ---- grouped digit product, 32-deep K slices -----------------------------------
+// ---- grouped digit product, 32 x 32 x 32 int8 MFMA -------------------------------
 // Work group: 2 x 2 waves, each a 32 x 32 block computed by
-// v_mfma_i32_32x32x32_i8 (64 x 64 output tile); K in 32-deep slices with
-// LDS double-buffered (all S planes of A and B, 2 x 32 KB for S = 8) so two
-// work groups share a CU, the next slice loaded into registers while the
-// current one is multiplied.  LDS keeps the digit groups in their global
+// v_mfma_i32_32x32x32_i8 (64 x 64 output tile), two work groups per CU.
+// K goes in stages of KS = 2 slices of 32: A's digit groups of a stage are
+// staged through LDS (double-buffered, 2 x 32 KB for S = 8) in their global
 // order (plane half j = 2 s + h, then row), so the loads, the LDS stores
 // (lane l: row l) and the fragment reads (lane l: row l % 32, half l / 32)
-// all move consecutive 16-byte slots.  The tiles of a product go 8 tile
-// rows at a time, so the 64 work groups that run together on one XCD share
-// 512 rows of A and 512 columns of B in its L2.  The per-scale int32 sums
-// are exact, so the result equals fm_ozaki_gemm's bit for bit.
+// all move consecutive 16-byte slots.  Each wave loads its B fragments
+// straight from global memory into registers (the two waves that share a
+// column block both load them; L2 serves the second), which halves the LDS
+// traffic per MFMA.  The products of a slice run by descending B digit t,
+// and the register of beta_t takes the next slice's beta_t right after its
+// last product, so every B load has most of a slice to land and B needs
+// one register set; the A half-stage loaded at the start of a slice is
+// stored to the free LDS buffer at its end.  One barrier per stage.  The
+// tiles of a product go 8 tile rows at a time, so the 64 work groups that
+// run together on one XCD share 512 rows of A and 512 columns of B in its
+// L2.  The per-scale int32 sums are exact, so the result equals
+// fm_ozaki_gemm's bit for bit.
 typedef int fm_i32x16 __attribute__((ext_vector_type(16)));
 
 template <int S>
 __global__ void __launch_bounds__(256, 2) fm_ozaki_gemm32(FmOzakiArgs args) {
-  constexpr int BM = 64, BN = 64, J = 2 * S, GROUP = 8;
-  constexpr int NCH = (J * BM + 255) / 256;  // 16-byte chunks per thread and side per slice
-  __shared__ __align__(16) fm_i32x4 As[2][J][BM];
-  __shared__ __align__(16) fm_i32x4 Bs[2][J][BN];
+  constexpr int KS = 2;
+  constexpr int BM = 64, J = 2 * S, JS = KS * J, GROUP = 8;
+  constexpr int NCH = (J * BM + 255) / 256;  // per 32-deep slice
+  __shared__ __align__(16) fm_i32x4 As[2][JS][BM];
 
   int wg = blockIdx.x;
   {
@@ -320,69 +327,83 @@ __global__ void __launch_bounds__(256, 2) fm_ozaki_gemm32(FmOzakiArgs args) {
   const int per_group = GROUP * d.tiles_n, grp = local / per_group;
   const int first = grp * GROUP, rows_g = min(GROUP, d.tiles_m - first);
   const int rem = local - grp * per_group;
-  const int m0 = (first + rem % rows_g) * BM, n0 = (rem / rows_g) * BN;
+  const int m0 = (first + rem % rows_g) * BM, n0 = (rem / rows_g) * 64;
   char* ws = reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]);
   const fm_i32x4* sa = reinterpret_cast<const fm_i32x4*>(ws + d.a_slc) + m0;
-  const fm_i32x4* sb = reinterpret_cast<const fm_i32x4*>(ws + d.b_slc) + n0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
+  const int fr = lane & 31, fh = lane >> 5;
   const int64_t pa = d.rows_pad, pb = d.cols_pad;
+  // this lane's B column (n0 + wn 32 + fr < cols_pad) and K half
+  const fm_i32x4* sbl = reinterpret_cast<const fm_i32x4*>(ws + d.b_slc) + n0 + wn * 32 + fr + fh * pb;
 
-  fm_i32x16 acc[S];  // one exact integer sum per scale group u = s + t + 2
+  fm_i32x16 acc[S];
 #pragma unroll
   for (int u = 0; u < S; ++u)
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[u][e] = 0;
 
-  fm_i32x4 ra[NCH], rb[NCH];
-  auto gload = [&](int kc) {
+  // A staging, one 32-deep slice h of stage kc at a time
+  fm_i32x4 ra[NCH];
+  auto gload = [&](int kc, int h) {
 #pragma unroll
     for (int u = 0; u < NCH; ++u) {
       const int o = tid + u * 256, row = o & (BM - 1), j = o / BM;
-      if ((J * BM) % 256 == 0 || o < J * BM) {
-        ra[u] = sa[(int64_t)(kc * J + j) * pa + row];
-        rb[u] = sb[(int64_t)(kc * J + j) * pb + row];
-      }
+      if ((J * BM) % 256 == 0 || o < J * BM) ra[u] = sa[(int64_t)(kc * JS + h * J + j) * pa + row];
     }
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&](int buf, int h) {
 #pragma unroll
     for (int u = 0; u < NCH; ++u) {
       const int o = tid + u * 256, row = o & (BM - 1), j = o / BM;
-      if ((J * BM) % 256 == 0 || o < J * BM) {
-        As[buf][j][row] = ra[u];
-        Bs[buf][j][row] = rb[u];
-      }
+      if ((J * BM) % 256 == 0 || o < J * BM) As[buf][h * J + j][row] = ra[u];
     }
   };
 
-  const int nkc = d.Kp >> 5;
-  const int fr = lane & 31, fh = lane >> 5;
+  const int nkc = d.Kp / (32 * KS);  // Kp is a multiple of 64
+  fm_i32x4 bf[S];
   if (nkc > 0) {
-    gload(0);
-    sstore(0);
+#pragma unroll
+    for (int h = 0; h < KS; ++h) {
+      gload(0, h);
+      sstore(0, h);
+    }
+#pragma unroll
+    for (int t = 0; t < S; ++t) bf[t] = sbl[(int64_t)(2 * t) * pb];
   }
   __syncthreads();
   for (int kc = 0; kc < nkc; ++kc) {
     const int buf = kc & 1;
-    if (kc + 1 < nkc) gload(kc + 1);
-    fm_i32x4 af[S], bf[S];
+    // The last stage reloads and restores itself (in bounds, never read)
+    // instead of branching: at a branch join the compiler would wait for
+    // every outstanding load, the fresh A loads included, before the first
+    // product.  Buffer buf ^ 1 is free: every wave finished reading it
+    // before the previous barrier.
+    const int kn = min(kc + 1, nkc - 1);
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      af[s] = As[buf][2 * s + fh][wm * 32 + fr];
-      bf[s] = Bs[buf][2 * s + fh][wn * 32 + fr];
+    for (int h = 0; h < KS; ++h) {
+      gload(kn, h);
+      // keep every load where it is written: the scheduler would sink them
+      // towards their uses, which leaves them no time to land
+      __builtin_amdgcn_sched_barrier(0);
+      // B of the next 32-deep slice
+      const int64_t nb = (int64_t)((h + 1 < KS ? kc * KS + h + 1 : kn * KS) * J) * pb;
+      fm_i32x4 af[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) af[s] = As[buf][h * J + 2 * s + fh][wm * 32 + fr];
+#pragma unroll
+      for (int t = S - 1; t >= 0; --t) {
+#pragma unroll
+        for (int s = 0; s + t < S; ++s)
+          acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[t], af[s], acc[s + t], 0, 0, 0);
+        bf[t] = sbl[nb + (int64_t)(2 * t) * pb];
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      sstore(buf ^ 1, h);
     }
-#pragma unroll
-    for (int s = 0; s < S; ++s)
-#pragma unroll
-      for (int t = 0; t + s < S; ++t)
-        acc[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[t], af[s], acc[s + t], 0, 0, 0);
-    if (kc + 1 < nkc) sstore(buf ^ 1);
     __syncthreads();
   }
 
-  // lane l holds C row (l % 32) of its block and, in element e, column
-  // (e % 4) + 4 (l / 32) + 8 (e / 4)
   const int32_t* ea = reinterpret_cast<const int32_t*>(ws + d.a_exp);
   const int32_t* eb = reinterpret_cast<const int32_t*>(ws + d.b_exp);
   double* C = reinterpret_cast<double*>(fm_base(args.bases, d.c.role)) + d.c.off;
@@ -394,7 +415,6 @@ __global__ void __launch_bounds__(256, 2) fm_ozaki_gemm32(FmOzakiArgs args) {
   for (int e = 0; e < 16; ++e) {
     const int col = n0 + wn * 32 + (e & 3) + 4 * fh + 8 * (e >> 2);
     if (col >= d.N) continue;
-    // smallest scale first: group u = s + t + 2 carries 2^-7u
     double sum = 0.0;
 #pragma unroll
     for (int u = S - 1; u >= 0; --u) sum = __dadd_rn(sum, ldexp((double)acc[u][e], -7 * (u + 2)));

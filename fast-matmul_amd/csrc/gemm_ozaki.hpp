@@ -23,7 +23,7 @@ struct FmOzakiArgs {
   const FmOzakiDesc* descs;
   int32_t ndesc;
   int32_t total_tiles;
-  int32_t variant;  // 0: 16 x 16 x 64 MFMA kernel, 64-deep slices; 1: 32 x 32 x 32, 32-deep slices
+  int32_t variant;  // 0: 16 x 16 x 64 MFMA kernel, 64-deep slices; 1: 32 x 32 x 32, two 32-deep slices per stage
   int32_t pad;
 };
 

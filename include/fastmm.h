@@ -161,9 +161,9 @@ fm_status fm_profile_read(fm_profile* out);
  * power-of-two scale, and S(S+1)/2 int8 products are summed. */
 fm_status fm_set_ozaki_slices(int slices);
 
-/* Tuning override: force leaf-GEMM tile configuration `cfg` (0 .. 27, see
+/* Tuning override: force leaf-GEMM tile configuration `cfg` (0 .. 30, see
  * FmTileCfg in the sources) for every plan built afterwards; -1 restores the
- * automatic choice.  Cached plans are dropped.  Configurations 24 .. 27 run
+ * automatic choice.  Cached plans are dropped.  Configurations 24 .. 29 run
  * split_bf16 leaves on prepared slice planes and are refused in other modes
  * (FM_ERR_INVALID_ARG when a plan is built); in ozaki_int8 mode, 1 (64 x 64)
  * selects the 16x16x64 digit kernel and any other value the default one. */
