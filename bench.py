@@ -89,8 +89,9 @@ def _err(C, D, A, B):
     return nw, float((d / D.abs()).max())
 
 
-def other_configs(fastmm, torch, dev):
-    """Secondary measurements of the other configurations, each guarded."""
+def other_configs(fastmm, torch, dev, names=("c1", "c2ts", "c3", "c4", "c5")):
+    """Secondary measurements of the other configurations (those in
+    `names`), each guarded."""
     sync = torch.cuda.synchronize
     out = {}
     g = torch.Generator(device=dev)
@@ -213,7 +214,8 @@ def other_configs(fastmm, torch, dev):
                                "normwise_err_max_first8": err_s}}
 
     for name, fn in (("c1", c1), ("c2ts", c2ts), ("c3", c3), ("c4", c4), ("c5", c5)):
-        guarded(name, fn)
+        if name in names:
+            guarded(name, fn)
     return out
 
 
@@ -227,6 +229,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=4096, help="square size of the CPU sample")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the secondary measurements of the other workloads")
+    ap.add_argument("--configs", default="c1,c2ts,c3,c4,c5",
+                    help="comma-separated secondary workloads to measure (c1, c2ts, c3, c4, c5)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB read + 1 GiB write elementwise kernel, the "
                          "known-size reference for HBM byte counters")
@@ -311,7 +315,7 @@ def main():
     if not args.no_configs and args.workload == "c2sq":
         del Cg, D, At, Bt
         torch.cuda.empty_cache()
-        configs = other_configs(fastmm, torch, dev)
+        configs = other_configs(fastmm, torch, dev, tuple(args.configs.split(",")))
 
     cpu = None
     if not args.no_cpu:
