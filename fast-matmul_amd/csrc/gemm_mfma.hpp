@@ -106,9 +106,10 @@ hipError_t fm_gemm_launch_bf16x3(int cfg, const FmGemmArgs& a, hipStream_t s);
 // An FP32 operand of `rows` x K (A: rows = M, row r = A[r, :]; B: rows = N,
 // row r = B[:, r]) is cut once into three bf16 slice planes in the
 // workspace.  Element (r, k), slice p (0 = leading) is the bf16 at byte
-//   dst + (r * Kp / 32 + k / 32) * 192 + p * 64 + (k % 32) * 2,
+//   dst + (k / 32 * rows_pad + r) * 192 + p * 64 + (k % 32) * 2,
 // so each 32-deep K slice of a row is one 192-byte chunk holding the three
-// planes; rows [rows, rows_pad) and k in [K, Kp) hold zeros.
+// planes, and the chunks of all rows of one K slice are consecutive; rows
+// [rows, rows_pad) and k in [K, Kp) hold zeros.
 struct FmSplitPrep {
   FmOp src;             // FP32 source, column-major
   int32_t rows, K;

@@ -239,7 +239,7 @@ hipError_t fm_gemm_launch_bf16x3(int cfg, const FmGemmArgs& a, hipStream_t s) {
 // ---- slice planes prepared once per operand -------------------------------------
 // One work group cuts 64 rows x one 32-deep K slice of one operand: each
 // thread 8 consecutive k of one row, the 4 threads of a row in adjacent
-// lanes, so a wave writes 16 whole 192-byte row chunks.  Sources with rows
+// lanes, so a wave writes 16 consecutive 192-byte row chunks.  Sources with rows
 // contiguous (A) load 64-byte runs down each column; sources with K
 // contiguous (B) 128-byte runs along each row.
 __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first) {
@@ -266,15 +266,16 @@ __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first
   }
   uint4 sl[3];
   fm_split3(x, sl);
-  char* dst = ws + d.dst + ((int64_t)row * (d.Kp >> 5) + kc) * 192 + q * 16;
+  char* dst = ws + d.dst + ((int64_t)kc * d.rows_pad + row) * 192 + q * 16;
 #pragma unroll
   for (int p = 0; p < 3; ++p) *reinterpret_cast<uint4*>(dst + p * 64) = sl[p];
 }
 
 // ---- grouped products on slice planes -------------------------------------------
-// K in 32-deep slices; each slice of a tile is BM (A) + BN (B) rows of 192
-// bytes, copied from the planes into LDS with 16-byte loads (no bounds
-// checks: the planes are padded to whole tiles and K slices).  LDS is
+// K in 32-deep slices; each slice of a tile is BM (A) + BN (B) consecutive
+// rows of 192 bytes, copied from the planes into LDS with 16-byte loads (no
+// bounds checks: the planes are padded to whole tiles and K slices, their
+// padded row counts tiles_m BM and tiles_n BN).  LDS is
 // double-buffered: the loads of slice kt + 1 are in flight while slice kt
 // is multiplied, then stored to the other buffer; one barrier per slice.
 // With DB false the LDS holds one slice (two work groups fit on a CU): the
@@ -332,9 +333,11 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
   const int m0 = (first + rem % rows_g) * BM, n0 = (rem / rows_g) * BN;
   const int nkc = d.Kp >> 5;
   const char* ws = reinterpret_cast<const char*>(args.bases.p[FM_ROLE_WS]);
-  // chunk o of a slice: plane o / (4 * rows), row (o / 4) % rows, chunk o % 4
-  const fm_u32x4* Ap = reinterpret_cast<const fm_u32x4*>(ws + d.a_pl) + (int64_t)m0 * nkc * 12;
-  const fm_u32x4* Bp = reinterpret_cast<const fm_u32x4*>(ws + d.b_pl) + (int64_t)n0 * nkc * 12;
+  // chunk o of a slice: plane o / (4 * rows), row (o / 4) % rows, chunk o % 4;
+  // consecutive K slices of a plane set are apa (bpa) rows apart
+  const int64_t apa = (int64_t)d.tiles_m * BM, bpa = (int64_t)d.tiles_n * BN;
+  const fm_u32x4* Ap = reinterpret_cast<const fm_u32x4*>(ws + d.a_pl) + (int64_t)m0 * 12;
+  const fm_u32x4* Bp = reinterpret_cast<const fm_u32x4*>(ws + d.b_pl) + (int64_t)n0 * 12;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
 
@@ -343,12 +346,12 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
 #pragma unroll
     for (int u = 0; u < A_CH; ++u) {
       const int o = tid + u * NT, c = o & 3, row = (o >> 2) % BM, p = (o >> 2) / BM;
-      ra[u] = Ap[((int64_t)row * nkc + kt) * 12 + p * 4 + c];
+      ra[u] = Ap[((int64_t)kt * apa + row) * 12 + p * 4 + c];
     }
 #pragma unroll
     for (int u = 0; u < B_CH; ++u) {
       const int o = tid + u * NT, c = o & 3, col = (o >> 2) % BN, p = (o >> 2) / BN;
-      rb[u] = Bp[((int64_t)col * nkc + kt) * 12 + p * 4 + c];
+      rb[u] = Bp[((int64_t)kt * bpa + col) * 12 + p * 4 + c];
     }
   };
   auto sstore = [&](int buf) {
