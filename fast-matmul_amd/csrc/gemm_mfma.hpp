@@ -69,11 +69,15 @@ enum FmTileCfg : int {
   FM_TILE_SP_128x256_IL = 29,  // as 26, fragment reads interleaved with the MFMAs
   // plain FP64 / FP32 again: 96-column strips (e.g. 2400 = 18 x 128 + 96)
   FM_TILE_PI_128x96_O2 = 30,   // BK 16, 2x2 waves of 64 x 48, registers for 2 waves per SIMD
-  FM_TILE_NUM = 31
+  // split-bf16 on slice planes, B fragments loaded straight into registers
+  FM_TILE_SP_128x256_BD = 31,  // 2x4 waves, A through double-buffered LDS
+  FM_TILE_NUM = 32
 };
 
 // configurations that run on prepared slice planes
-inline bool fm_tile_planes(int cfg) { return cfg >= FM_TILE_SP_128x128 && cfg <= FM_TILE_SP_128x256_IL; }
+inline bool fm_tile_planes(int cfg) {
+  return (cfg >= FM_TILE_SP_128x128 && cfg <= FM_TILE_SP_128x256_IL) || cfg == FM_TILE_SP_128x256_BD;
+}
 
 // K slice depth of each configuration
 inline int fm_tile_bk(int cfg) {
@@ -92,7 +96,7 @@ inline void fm_tile_dims(int cfg, int* bm, int* bn) {
                                         {128, 128}, {128, 256}, {256, 128}, {128, 256},
                                         {128, 128}, {128, 128}, {128, 128}, {128, 256},
                                         {128, 128}, {256, 128}, {128, 256}, {128, 128},
-                                        {256, 128}, {128, 256}, {128, 96}};
+                                        {256, 128}, {128, 256}, {128, 96},  {128, 256}};
   *bm = d[cfg][0];
   *bn = d[cfg][1];
 }

@@ -461,6 +461,155 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
   }  // tile loop
 }
 
+// fm_split_gemm with the B fragments loaded by each wave straight from the
+// planes into registers (the WM waves that share a column block each load
+// it; L2 serves the repeats) and only A staged through LDS (double-
+// buffered): less LDS traffic per MFMA.  The products of a slice run by
+// column block j, and the registers of block j take the next slice's B
+// fragments right after its last product, so every B load has most of a
+// slice to land with one register set.  The slice products of each output
+// fragment and their order are fm_split_gemm's, so the results are
+// bit-identical.  One tile per work group (launched with one work group per
+// tile).
+template <int BM, int BN, int WM, int WN>
+__global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm_bd(FmSplitArgs args) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int WTM = BM / WM, WTN = BN / WN;
+  constexpr int TM = WTM / 16, TN = WTN / 16;
+  constexpr int A_CH = BM * 12 / NT;  // 16-byte chunks per thread and slice
+  static_assert(A_CH * NT == BM * 12, "tile/thread mismatch");
+  static_assert(TM >= 1 && TN >= 1, "bad tile");
+  __shared__ __align__(16) fm_u32x4 As[2][3][BM][4];
+
+  const int wg = fm_xcd_remap2(blockIdx.x, args.total_tiles);
+  int g;
+  if (args.uniform) {
+    g = wg / (args.descs[0].tiles_m * args.descs[0].tiles_n);
+  } else {
+    int lo = 0, hi = args.ndesc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (args.descs[mid].tile_begin <= wg)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    g = lo;
+  }
+  const FmSplitDesc d = args.descs[g];
+  constexpr int SP_GROUP = BM == 256 ? 4 : 8;
+  const int local = wg - d.tile_begin;
+  const int per_group = SP_GROUP * d.tiles_n, grp = local / per_group;
+  const int first = grp * SP_GROUP, rows_g = min(SP_GROUP, d.tiles_m - first);
+  const int rem = local - grp * per_group;
+  const int m0 = (first + rem % rows_g) * BM, n0 = (rem / rows_g) * BN;
+  const int nkc = d.Kp >> 5;
+  const char* ws = reinterpret_cast<const char*>(args.bases.p[FM_ROLE_WS]);
+  const int64_t apa = (int64_t)d.tiles_m * BM, bpa = (int64_t)d.tiles_n * BN;
+  const fm_u32x4* Ap = reinterpret_cast<const fm_u32x4*>(ws + d.a_pl) + (int64_t)m0 * 12;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int fr = lane & 15, fc = lane >> 4;
+  const int fsw = fc ^ fm_sp_swz(fr);
+  // this lane's B fragment bytes: column n0 + wn WTN + 16 j + fr, chunk fc
+  // of each plane (bf16 k = 8 fc .. 8 fc + 7 of the slice)
+  const char* bl = ws + d.b_pl + (int64_t)(n0 + wn * WTN + fr) * 192 + fc * 16;
+
+  fm_u32x4 ra[A_CH];
+  auto gload = [&](int kt) {
+#pragma unroll
+    for (int u = 0; u < A_CH; ++u) {
+      const int o = tid + u * NT, c = o & 3, row = (o >> 2) % BM, p = (o >> 2) / BM;
+      ra[u] = Ap[((int64_t)kt * apa + row) * 12 + p * 4 + c];
+    }
+  };
+  auto sstore = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < A_CH; ++u) {
+      const int o = tid + u * NT, c = o & 3, row = (o >> 2) % BM, p = (o >> 2) / BM;
+      As[buf][p][row][c ^ fm_sp_swz(row)] = ra[u];
+    }
+  };
+  fm_bf16x8 bf[3][TN];
+  auto bload = [&](int kt, int j) {
+#pragma unroll
+    for (int p = 0; p < 3; ++p)
+      bf[p][j] = *reinterpret_cast<const fm_bf16x8*>(bl + ((int64_t)kt * bpa + 16 * j) * 192 + p * 64);
+  };
+
+  fm_f32x4 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j) acc[i][j] = fm_f32x4{0.f, 0.f, 0.f, 0.f};
+
+  if (nkc > 0) {
+    gload(0);
+    sstore(0);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) bload(0, j);
+  }
+  __syncthreads();
+  for (int kt = 0; kt < nkc; ++kt) {
+    const int buf = kt & 1;
+    // The last slice reloads and restores itself (in bounds, never read)
+    // instead of branching: at a branch join the compiler would wait for
+    // every outstanding load before the first product.  Buffer buf ^ 1 is
+    // free: every wave finished reading it before the previous barrier.
+    const int kn = min(kt + 1, nkc - 1);
+    gload(kn);
+    // keep every load where it is written: the scheduler would sink them
+    // towards their uses, which leaves them no time to land
+    __builtin_amdgcn_sched_barrier(0);
+    fm_bf16x8 af[3][TM];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int p = 0; p < 3; ++p)
+        af[p][i] = *reinterpret_cast<const fm_bf16x8*>(&As[buf][p][wm * WTM + i * 16 + fr][fsw]);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i) {
+        fm_f32x4 c = acc[i][j];
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[2][j], af[0][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[1][j], af[1][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[2][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[1][j], af[0][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[1][i], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[0][i], c, 0, 0, 0);
+        acc[i][j] = c;
+      }
+      bload(kn, j);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    sstore(buf ^ 1);
+    __syncthreads();
+  }
+
+  const float alpha = (float)d.alpha, beta = (float)d.beta;
+  float* __restrict__ C = reinterpret_cast<float*>(fm_base(args.bases, d.c.role)) + d.c.off;
+  const int64_t ldc = d.c.ld;
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int row = m0 + wm * WTM + i * 16 + fr;
+    if (row >= d.M) continue;
+#pragma unroll
+    for (int j = 0; j < TN; ++j) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int col = n0 + wn * WTN + j * 16 + 4 * fc + v;
+        if (col < d.N) {
+          float* p = C + row + (int64_t)col * ldc;
+          float r = alpha * acc[i][j][v];
+          if (beta != 0.f) r += beta * *p;
+          *p = r;
+        }
+      }
+    }
+  }
+}
+
 hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, hipStream_t s) {
   if (a.nprep > 0 && a.prep_bx > 0 && a.prep_by > 0) {
     for (int first = 0; first < a.nprep; first += 65535) {
@@ -473,6 +622,9 @@ hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, hipStream_t s) {
   if (a.total_tiles <= 0) return hipSuccess;
   const int grid = a.grid > 0 && a.grid < a.total_tiles ? a.grid : a.total_tiles;
   switch (cfg) {
+    case FM_TILE_SP_128x256_BD:  // one work group per tile
+      hipLaunchKernelGGL((fm_split_gemm_bd<128, 256, 2, 4>), dim3(a.total_tiles), dim3(512), 0, s, a);
+      break;
     case FM_TILE_SP_256x128:
       hipLaunchKernelGGL((fm_split_gemm<256, 128, 4, 2, true>), dim3(grid), dim3(512), 0, s, a);
       break;

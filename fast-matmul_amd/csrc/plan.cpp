@@ -347,10 +347,13 @@ const TileCand kF32Tiles[] = {
     {FM_TILE_64x64, FM_TILE_64x64, 0.66, 4}};
 // split-bf16: kernels on prepared slice planes (any shape: the planes are
 // padded to whole tiles)
-const TileCand kSplitTiles[] = {{FM_TILE_SP_256x128_IL, FM_TILE_SP_256x128_IL, 1.00, 1},
-                                {FM_TILE_SP_128x256_IL, FM_TILE_SP_128x256_IL, 1.00, 1},
-                                {FM_TILE_SP_128x128, FM_TILE_SP_128x128, 0.95, 2},
-                                {FM_TILE_SP_128x128_DB, FM_TILE_SP_128x128_DB, 0.90, 1}};
+// (C4, 49 products of 4096^3: B straight to registers 251, the LDS-staged
+// 128 x 256 and 256 x 128 kernels 232 and 229 TFLOP/s)
+const TileCand kSplitTiles[] = {{FM_TILE_SP_128x256_BD, FM_TILE_SP_128x256_BD, 1.00, 1},
+                                {FM_TILE_SP_256x128_IL, FM_TILE_SP_256x128_IL, 0.92, 1},
+                                {FM_TILE_SP_128x256_IL, FM_TILE_SP_128x256_IL, 0.92, 1},
+                                {FM_TILE_SP_128x128, FM_TILE_SP_128x128, 0.88, 2},
+                                {FM_TILE_SP_128x128_DB, FM_TILE_SP_128x128_DB, 0.83, 1}};
 // edge kernels run slower than whole-tile ones (predicated loads)
 const double kEdgePenalty = 0.93;
 

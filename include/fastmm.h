@@ -86,8 +86,9 @@ fm_status fm_gemm(fm_dtype dtype, fm_mode mode, int m, int k, int n, const void*
 
 /* ---- batches of independent multiplies -------------------------------------
  * Many C_i := alpha_i A_i B_i + beta_i C_i in one launch sequence: every
- * problem's S/T stages, ONE grouped launch of all base-case products, then
- * the W stages and one grouped peeling launch per level.  A problem with
+ * problem's S/T stages, ONE grouped launch of all base-case products (plus
+ * one or two for edge strips and remainder tiles), then the W stages and one
+ * grouped peeling launch per level.  A problem with
  * alg == NULL (or num_steps < 0) gets its algorithm (from `candidates`,
  * default "strassen", "grey423_20_144" in its six orientations
  * (grey432/243/234/324/342_20_144) and "smirnov333_23_139") and depth
@@ -161,9 +162,9 @@ fm_status fm_profile_read(fm_profile* out);
  * power-of-two scale, and S(S+1)/2 int8 products are summed. */
 fm_status fm_set_ozaki_slices(int slices);
 
-/* Tuning override: force leaf-GEMM tile configuration `cfg` (0 .. 30, see
+/* Tuning override: force leaf-GEMM tile configuration `cfg` (0 .. 31, see
  * FmTileCfg in the sources) for every plan built afterwards; -1 restores the
- * automatic choice.  Cached plans are dropped.  Configurations 24 .. 29 run
+ * automatic choice.  Cached plans are dropped.  Configurations 24 .. 29 and 31 run
  * split_bf16 leaves on prepared slice planes and are refused in other modes
  * (FM_ERR_INVALID_ARG when a plan is built); in ozaki_int8 mode, 1 (64 x 64)
  * selects the 16x16x64 digit kernel and any other value the default one. */

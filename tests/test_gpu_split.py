@@ -102,7 +102,7 @@ def test_split_accuracy_fp32_level(fm, oracle_mod, dist, alg, steps):
 
 # FmTileCfg: 0 = in-loop split kernel (128 x 128), 24-29 = kernels on
 # prepared slice planes (28, 29: fragment reads interleaved with the MFMAs)
-INLOOP, PLANE_CFGS = 0, (24, 25, 26, 27, 28, 29)
+INLOOP, PLANE_CFGS = 0, (24, 25, 26, 27, 28, 29, 31)
 
 
 @pytest.fixture

@@ -87,7 +87,7 @@ def test_split_leaves_on_slice_planes():
     the tile, K to 32), on top of the plain FP32 plan's workspace."""
     p = fastmm.plan_info("strassen", 4096, 4096, 4096, 1, dtype=torch.float32, mode="split_bf16")
     q = fastmm.plan_info("strassen", 4096, 4096, 4096, 1, dtype=torch.float32)
-    assert p["leaf"][0]["cfg"] in (24, 25, 26, 27, 28, 29)
+    assert p["leaf"][0]["cfg"] in (24, 25, 26, 27, 28, 29, 31)
     # Strassen level 1: 7 products of 2048^3, each with its own A-side
     # operand (5 sums, the blocks A11 and A22) and B-side operand: 14 planes
     planes = p["workspace_bytes"] - q["workspace_bytes"]
