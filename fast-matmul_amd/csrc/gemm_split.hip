@@ -237,38 +237,49 @@ hipError_t fm_gemm_launch_bf16x3(int cfg, const FmGemmArgs& a, hipStream_t s) {
 }
 
 // ---- slice planes prepared once per operand -------------------------------------
-// One work group cuts 64 rows x one 32-deep K slice of one operand: each
-// thread 8 consecutive k of one row, the 4 threads of a row in adjacent
-// lanes, so a wave writes 16 consecutive 192-byte row chunks.  Sources with rows
-// contiguous (A) load 64-byte runs down each column; sources with K
-// contiguous (B) 128-byte runs along each row.
+// One work group cuts 64 rows x FM_PREP_KCH consecutive 32-deep K slices of
+// one operand: per slice each thread 8 consecutive k of one row, the 4
+// threads of a row in adjacent lanes, so a wave writes 16 consecutive
+// 192-byte row chunks.  Sources with rows contiguous (A) load 64-byte runs
+// down each column; sources with K contiguous (B) 128-byte runs along each
+// row.  The grid covers the largest operand of the launch; several slices
+// per work group spread its descriptor load over more work and keep the
+// grid small when the operands differ in size (a work group past its
+// operand's end exits after that load).
+constexpr int FM_PREP_KCH = 4;
+
 __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first) {
   const FmSplitPrep d = args.preps[first + blockIdx.z];
-  const int r0 = blockIdx.x * 64, kc = blockIdx.y;
-  if (r0 >= d.rows_pad || kc * 32 >= d.Kp) return;
+  const int r0 = blockIdx.x * 64, kc0 = blockIdx.y * FM_PREP_KCH;
+  const int nkc = d.Kp >> 5;
+  if (r0 >= d.rows_pad || kc0 >= nkc) return;
   const float* __restrict__ src =
       reinterpret_cast<const float*>(fm_base(args.bases, d.src.role)) + d.src.off;
   char* ws = reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]);
   const int tid = threadIdx.x;
   const int r = tid >> 2, q = tid & 3;
-  const int row = r0 + r, k = kc * 32 + 8 * q;
+  const int row = r0 + r;
   const bool rok = row < d.rows;
   const int64_t ld = d.src.ld;
-  float x[8];
-  if (d.kcontig) {
-    const float* p = src + (int64_t)row * ld + k;
+  const int kc1 = min(kc0 + FM_PREP_KCH, nkc);
+  for (int kc = kc0; kc < kc1; ++kc) {
+    const int k = kc * 32 + 8 * q;
+    float x[8];
+    if (d.kcontig) {
+      const float* p = src + (int64_t)row * ld + k;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) x[e] = (rok && k + e < d.K) ? p[e] : 0.f;
-  } else {
-    const float* p = src + row + (int64_t)k * ld;
+      for (int e = 0; e < 8; ++e) x[e] = (rok && k + e < d.K) ? p[e] : 0.f;
+    } else {
+      const float* p = src + row + (int64_t)k * ld;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) x[e] = (rok && k + e < d.K) ? p[(int64_t)e * ld] : 0.f;
+      for (int e = 0; e < 8; ++e) x[e] = (rok && k + e < d.K) ? p[(int64_t)e * ld] : 0.f;
+    }
+    uint4 sl[3];
+    fm_split3(x, sl);
+    char* dst = ws + d.dst + ((int64_t)kc * d.rows_pad + row) * 192 + q * 16;
+#pragma unroll
+    for (int p = 0; p < 3; ++p) *reinterpret_cast<uint4*>(dst + p * 64) = sl[p];
   }
-  uint4 sl[3];
-  fm_split3(x, sl);
-  char* dst = ws + d.dst + ((int64_t)kc * d.rows_pad + row) * 192 + q * 16;
-#pragma unroll
-  for (int p = 0; p < 3; ++p) *reinterpret_cast<uint4*>(dst + p * 64) = sl[p];
 }
 
 // ---- grouped products on slice planes -------------------------------------------
@@ -614,7 +625,9 @@ hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, hipStream_t s) {
   if (a.nprep > 0 && a.prep_bx > 0 && a.prep_by > 0) {
     for (int first = 0; first < a.nprep; first += 65535) {
       const int nz = a.nprep - first < 65535 ? a.nprep - first : 65535;
-      hipLaunchKernelGGL(fm_split_prep, dim3(a.prep_bx, a.prep_by, nz), dim3(256), 0, s, a, first);
+      hipLaunchKernelGGL(fm_split_prep,
+                         dim3(a.prep_bx, (a.prep_by + FM_PREP_KCH - 1) / FM_PREP_KCH, nz),
+                         dim3(256), 0, s, a, first);
     }
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
