@@ -140,7 +140,7 @@ struct FmSplitArgs {
   int32_t uniform, grid;        // as in FmGemmArgs
   int32_t nprep;
   int32_t prep_bx, prep_by;     // prep grid: row blocks of 64, K slices of 32
-  int32_t pad;
+  int32_t even_k;               // 1: every product has an even number of K slices
 };
 
 // the preparation launch, then the grouped product launch

@@ -480,9 +480,12 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
 // fragments right after its last product, so every B load has most of a
 // slice to land with one register set.  The slice products of each output
 // fragment and their order are fm_split_gemm's, so the results are
-// bit-identical.  One tile per work group (launched with one work group per
-// tile).
-template <int BM, int BN, int WM, int WN>
+// bit-identical.  KS 32-deep slices share one LDS stage and one barrier
+// (KS = 2 only when every product has an even number of slices): the A
+// slice h of the next stage loads at the start of slice h and is stored to
+// the free buffer at its end.  One tile per work group (launched with one
+// work group per tile).
+template <int BM, int BN, int WM, int WN, int KS>
 __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm_bd(FmSplitArgs args) {
   constexpr int NT = 64 * WM * WN;
   constexpr int WTM = BM / WM, WTN = BN / WN;
@@ -490,7 +493,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm_bd(FmSplitArgs arg
   constexpr int A_CH = BM * 12 / NT;  // 16-byte chunks per thread and slice
   static_assert(A_CH * NT == BM * 12, "tile/thread mismatch");
   static_assert(TM >= 1 && TN >= 1, "bad tile");
-  __shared__ __align__(16) fm_u32x4 As[2][3][BM][4];
+  __shared__ __align__(16) fm_u32x4 As[2][KS][3][BM][4];
 
   const int wg = fm_xcd_remap2(blockIdx.x, args.total_tiles);
   int g;
@@ -527,18 +530,18 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm_bd(FmSplitArgs arg
   const char* bl = ws + d.b_pl + (int64_t)(n0 + wn * WTN + fr) * 192 + fc * 16;
 
   fm_u32x4 ra[A_CH];
-  auto gload = [&](int kt) {
+  auto gload = [&](int ks) {  // A of 32-deep slice ks
 #pragma unroll
     for (int u = 0; u < A_CH; ++u) {
       const int o = tid + u * NT, c = o & 3, row = (o >> 2) % BM, p = (o >> 2) / BM;
-      ra[u] = Ap[((int64_t)kt * apa + row) * 12 + p * 4 + c];
+      ra[u] = Ap[((int64_t)ks * apa + row) * 12 + p * 4 + c];
     }
   };
-  auto sstore = [&](int buf) {
+  auto sstore = [&](int buf, int h) {
 #pragma unroll
     for (int u = 0; u < A_CH; ++u) {
       const int o = tid + u * NT, c = o & 3, row = (o >> 2) % BM, p = (o >> 2) / BM;
-      As[buf][p][row][c ^ fm_sp_swz(row)] = ra[u];
+      As[buf][h][p][row][c ^ fm_sp_swz(row)] = ra[u];
     }
   };
   fm_bf16x8 bf[3][TN];
@@ -554,47 +557,56 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm_bd(FmSplitArgs arg
 #pragma unroll
     for (int j = 0; j < TN; ++j) acc[i][j] = fm_f32x4{0.f, 0.f, 0.f, 0.f};
 
-  if (nkc > 0) {
-    gload(0);
-    sstore(0);
+  const int nst = nkc / KS;  // stages (nkc is a multiple of KS)
+  if (nst > 0) {
+#pragma unroll
+    for (int h = 0; h < KS; ++h) {
+      gload(h);
+      sstore(0, h);
+    }
 #pragma unroll
     for (int j = 0; j < TN; ++j) bload(0, j);
   }
   __syncthreads();
-  for (int kt = 0; kt < nkc; ++kt) {
+  for (int kt = 0; kt < nst; ++kt) {
     const int buf = kt & 1;
-    // The last slice reloads and restores itself (in bounds, never read)
+    // The last stage reloads and restores itself (in bounds, never read)
     // instead of branching: at a branch join the compiler would wait for
     // every outstanding load before the first product.  Buffer buf ^ 1 is
     // free: every wave finished reading it before the previous barrier.
-    const int kn = min(kt + 1, nkc - 1);
-    gload(kn);
-    // keep every load where it is written: the scheduler would sink them
-    // towards their uses, which leaves them no time to land
-    __builtin_amdgcn_sched_barrier(0);
-    fm_bf16x8 af[3][TM];
+    const int kn = min(kt + 1, nst - 1);
 #pragma unroll
-    for (int i = 0; i < TM; ++i)
-#pragma unroll
-      for (int p = 0; p < 3; ++p)
-        af[p][i] = *reinterpret_cast<const fm_bf16x8*>(&As[buf][p][wm * WTM + i * 16 + fr][fsw]);
-#pragma unroll
-    for (int j = 0; j < TN; ++j) {
-#pragma unroll
-      for (int i = 0; i < TM; ++i) {
-        fm_f32x4 c = acc[i][j];
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[2][j], af[0][i], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[1][j], af[1][i], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[2][i], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[1][j], af[0][i], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[1][i], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[0][i], c, 0, 0, 0);
-        acc[i][j] = c;
-      }
-      bload(kn, j);
+    for (int h = 0; h < KS; ++h) {
+      gload(kn * KS + h);
+      // keep every load where it is written: the scheduler would sink them
+      // towards their uses, which leaves them no time to land
       __builtin_amdgcn_sched_barrier(0);
+      // B of the next 32-deep slice
+      const int nb = h + 1 < KS ? kt * KS + h + 1 : kn * KS;
+      fm_bf16x8 af[3][TM];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int p = 0; p < 3; ++p)
+          af[p][i] = *reinterpret_cast<const fm_bf16x8*>(&As[buf][h][p][wm * WTM + i * 16 + fr][fsw]);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          fm_f32x4 c = acc[i][j];
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[2][j], af[0][i], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[1][j], af[1][i], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[2][i], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[1][j], af[0][i], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[1][i], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bf[0][j], af[0][i], c, 0, 0, 0);
+          acc[i][j] = c;
+        }
+        bload(nb, j);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      sstore(buf ^ 1, h);
     }
-    sstore(buf ^ 1);
     __syncthreads();
   }
 
@@ -636,7 +648,10 @@ hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, hipStream_t s) {
   const int grid = a.grid > 0 && a.grid < a.total_tiles ? a.grid : a.total_tiles;
   switch (cfg) {
     case FM_TILE_SP_128x256_BD:  // one work group per tile
-      hipLaunchKernelGGL((fm_split_gemm_bd<128, 256, 2, 4>), dim3(a.total_tiles), dim3(512), 0, s, a);
+      if (a.even_k)
+        hipLaunchKernelGGL((fm_split_gemm_bd<128, 256, 2, 4, 2>), dim3(a.total_tiles), dim3(512), 0, s, a);
+      else
+        hipLaunchKernelGGL((fm_split_gemm_bd<128, 256, 2, 4, 1>), dim3(a.total_tiles), dim3(512), 0, s, a);
       break;
     case FM_TILE_SP_256x128:
       hipLaunchKernelGGL((fm_split_gemm<256, 128, 4, 2, true>), dim3(grid), dim3(512), 0, s, a);

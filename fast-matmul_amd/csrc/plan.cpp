@@ -284,6 +284,9 @@ Launch split_launch(std::vector<FmGemmDesc> prods, int cfg, size_t* ws_bytes) {
   L.sp.nprep = (int32_t)L.spreps.size();
   L.sp.prep_bx = max_rb;
   L.sp.prep_by = max_kc;
+  L.sp.even_k = 1;
+  for (const FmSplitDesc& d : L.sdescs)
+    if ((d.Kp / 32) % 2 != 0) L.sp.even_k = 0;
   *ws_bytes = off;
   return L;
 }
