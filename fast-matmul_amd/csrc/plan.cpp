@@ -350,8 +350,9 @@ const TileCand kF32Tiles[] = {
     {FM_TILE_64x64, FM_TILE_64x64, 0.66, 4}};
 // split-bf16: kernels on prepared slice planes (any shape: the planes are
 // padded to whole tiles)
-// (C4, 49 products of 4096^3: B straight to registers 251, the LDS-staged
-// 128 x 256 and 256 x 128 kernels 232 and 229 TFLOP/s)
+// (Strassen, two levels, 16384^3 FP32, i.e. 49 products of 4096^3, whole
+// multiply: 251 TFLOP/s with B straight to registers, 232 and 229 with the
+// LDS-staged 128 x 256 and 256 x 128 kernels)
 const TileCand kSplitTiles[] = {{FM_TILE_SP_128x256_BD, FM_TILE_SP_128x256_BD, 1.00, 1},
                                 {FM_TILE_SP_256x128_IL, FM_TILE_SP_256x128_IL, 0.92, 1},
                                 {FM_TILE_SP_128x256_IL, FM_TILE_SP_128x256_IL, 0.92, 1},
