@@ -34,6 +34,18 @@ def test_bench_workload_plan():
     assert p["leaf"] == [{"cfg": 22, "products": 676, "tiles": 10816}]
 
 
+def test_split_workload_plan():
+    # Strassen, 2 levels, 16384^3 FP32 with split-bf16 leaves: 49 leaves of
+    # 4096^3 on the kernel that loads B straight into registers (FmTileCfg
+    # 31, 128 x 256 tiles): 49 x 32 x 16 = 25088 tiles in one launch, the
+    # slice planes of 98 distinct operands (rows x K x 6 bytes each) on top
+    # of the plain FP32 plan's workspace
+    p = fastmm.plan_info("strassen", 16384, 16384, 16384, 2, dtype=torch.float32, mode="split_bf16")
+    q = fastmm.plan_info("strassen", 16384, 16384, 16384, 2, dtype=torch.float32)
+    assert p["leaf"] == [{"cfg": 31, "products": 49, "tiles": 25088}]
+    assert p["workspace_bytes"] - q["workspace_bytes"] == 98 * 4096 * 4096 * 6
+
+
 def test_one_level_stages_when_middle_level_peels():
     p = fastmm.plan_info("strassen", 130, 130, 130, 2)
     assert p["two_level_stages"] == 0
