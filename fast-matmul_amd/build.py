@@ -52,13 +52,27 @@ HOST = ["-O2", "-fPIC", "-std=c++17", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/inc
         "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-Wall", "-Wno-unused-function"]
 
 
-def compile_one(src, extra):
+def _up_to_date(obj, dep):
+    """True when obj and its dependency list exist and no dependency is newer."""
+    if not (os.path.exists(obj) and os.path.exists(dep)):
+        return False
+    t = os.path.getmtime(obj)
+    with open(dep) as f:
+        text = f.read().replace("\\\n", " ")
+    files = text.split(":", 1)[1].split() if ":" in text else []
+    return all(os.path.exists(d) and os.path.getmtime(d) <= t for d in files)
+
+
+def compile_one(src, extra, incremental=False):
     rel = os.path.relpath(src, CSRC).replace(os.sep, "__")
     obj = os.path.join(BUILD, rel + ".o")
+    dep = obj + ".d"
+    if incremental and _up_to_date(obj, dep):
+        return obj, 0.0
     if src.endswith(".hip"):
-        cmd = [HIPCC] + COMMON + extra + ["-x", "hip", "-c", src, "-o", obj]
+        cmd = [HIPCC] + COMMON + extra + ["-MMD", "-MF", dep, "-x", "hip", "-c", src, "-o", obj]
     else:
-        cmd = [HOSTCXX] + HOST + ["-c", src, "-o", obj]
+        cmd = [HOSTCXX] + HOST + ["-MMD", "-MF", dep, "-c", src, "-o", obj]
     t0 = time.time()
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if r.returncode != 0:
@@ -66,7 +80,9 @@ def compile_one(src, extra):
     return obj, time.time() - t0
 
 
-def build(jobs: int = 0, verbose: bool = False) -> str:
+def build(jobs: int = 0, verbose: bool = False, incremental: bool = False) -> str:
+    """Generate and compile everything (incremental: skip objects whose
+    sources and headers are unchanged since they were built)."""
     sys.path.insert(0, os.path.join(PKG, "gen"))
     import hipgen  # noqa: E402
 
@@ -76,7 +92,7 @@ def build(jobs: int = 0, verbose: bool = False) -> str:
     jobs = jobs or min(16, os.cpu_count() or 4)
     objs = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [ex.submit(compile_one, s, e) for s, e in srcs]
+        futs = [ex.submit(compile_one, s, e, incremental) for s, e in srcs]
         for f in futs:
             obj, dt = f.result()
             objs.append(obj)
@@ -95,6 +111,8 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("-j", type=int, default=0)
     ap.add_argument("-v", action="store_true")
+    ap.add_argument("--incremental", action="store_true",
+                    help="recompile only sources whose dependencies changed")
     a = ap.parse_args()
     t0 = time.time()
-    print(build(a.j, a.v), "%.1fs" % (time.time() - t0))
+    print(build(a.j, a.v, a.incremental), "%.1fs" % (time.time() - t0))

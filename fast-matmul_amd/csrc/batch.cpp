@@ -160,7 +160,10 @@ fm_status fm_batch_create(fm_dtype dtype, fm_mode mode, const fm_problem* proble
   Plan& P = b->plan;
   P.dtype = dtype;
   P.mode = mode;
-  P.slices = mode == FM_MODE_OZAKI_INT8 ? g_ozaki_slices : 0;
+  const fm_options opt = g_defaults;
+  P.slices = mode == FM_MODE_OZAKI_INT8 ? opt.ozaki_slices : 0;
+  P.tile = opt.gemm_tile;
+  P.split_rem = opt.split_remainder != 0;
   std::vector<SubPlan> subs(count);
   int64_t ws = 0;
   for (int i = 0; i < count; ++i) {
@@ -208,7 +211,11 @@ fm_status fm_batch_create(fm_dtype dtype, fm_mode mode, const fm_problem* proble
     p.role_c = 6 + 3 * i;
     p.alpha = q.alpha;
     p.beta = q.beta;
-    p.fuse = g_stage_fusion != 0;
+    p.fuse = opt.stage_fusion != 0;
+    p.mode = mode;
+    p.slices = P.slices;
+    p.tile = P.tile;
+    p.split_rem = P.split_rem;
     P.ext.push_back(const_cast<void*>(q.A));
     P.ext.push_back(const_cast<void*>(q.B));
     P.ext.push_back(q.C);

@@ -175,7 +175,8 @@ Launch gemm_launch(std::vector<FmGemmDesc> descs, int cfg, int stage, fm_dtype d
 
 // Leaf products on int8 digits: descriptors plus digit-plane workspace,
 // appended at byte offset *ws_bytes (advanced).
-fm_status ozaki_launch(int S, const std::vector<FmGemmDesc>& prods, size_t* ws_bytes, Launch& L) {
+fm_status ozaki_launch(int S, int tile, const std::vector<FmGemmDesc>& prods, size_t* ws_bytes,
+                       Launch& L) {
   int32_t t = 0;
   int max_rows = 0, max_cols = 0;
   size_t off = (size_t)align_up((int64_t)*ws_bytes, 256);
@@ -216,7 +217,7 @@ fm_status ozaki_launch(int S, const std::vector<FmGemmDesc>& prods, size_t* ws_b
   L.oz.ndesc = (int32_t)L.odescs.size();
   L.oz.total_tiles = t;
   // the 64 x 64 tile override selects the 16 x 16 x 64 kernel
-  L.oz.variant = g_tile_override == FM_TILE_64x64 ? 0 : 1;
+  L.oz.variant = tile == FM_TILE_64x64 ? 0 : 1;
   L.oz_slices = S;
   L.oz_rows = max_rows;
   L.oz_cols = max_cols;
@@ -293,11 +294,15 @@ Launch split_launch(std::vector<FmGemmDesc> prods, int cfg, size_t* ws_bytes) {
 
 }  // namespace
 
-int g_tile_override = -1;
-int g_ozaki_slices = 8;
-int g_stage_fusion = 1;
-int g_dfs_levels = 0;
-size_t g_ws_limit = 0;
+fm_options g_defaults = {-1, 0, 0, 1, 1, 8};
+
+fm_status check_options(const fm_options& o) {
+  if (o.gemm_tile < -1 || o.gemm_tile >= FM_TILE_NUM) return set_err(FM_ERR_INVALID_ARG, "unknown tile config");
+  if (o.dfs_levels < -1) return set_err(FM_ERR_INVALID_ARG, "dfs_levels must be >= -1");
+  if (o.ozaki_slices < 6 || o.ozaki_slices > 8)
+    return set_err(FM_ERR_INVALID_ARG, "ozaki_slices must be 6, 7 or 8");
+  return FM_OK;
+}
 
 fm_status set_err(fm_status s, const std::string& msg) {
   t_err = msg;
@@ -450,7 +455,7 @@ FmGemmDesc gsub(const FmGemmDesc& d, int64_t i0, int64_t j0, int64_t m, int64_t 
 
 bool carve_full(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods, std::vector<FmGemmDesc>& strips,
                 int* cfg_core, int* cfg_strips) {
-  if (g_tile_override >= 0 || P.mode != FM_MODE_PLAIN) return false;
+  if (P.tile >= 0 || P.mode != FM_MODE_PLAIN) return false;
   const TileCand* c = find_cand(P.dtype, P.mode, cfg);
   if (!c || whole_tiles(P.dtype, c->cfg_full, prods)) return false;
   int bm, bn;
@@ -479,10 +484,8 @@ bool carve_full(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods, std::vec
 }
 }  // namespace
 
-int g_split_remainder = 1;
-
-int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods) {
-  if (g_tile_override >= 0) return g_tile_override;
+int pick_tile(fm_dtype dt, fm_mode mode, int tile, const std::vector<FmGemmDesc>& prods) {
+  if (tile >= 0) return tile;
   return best_tile(dt, mode, prods, nullptr);
 }
 
@@ -494,7 +497,7 @@ int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods) {
 // every output element is computed by the same instructions either way.
 void split_remainder(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods,
                      std::vector<FmGemmDesc>& rest, int* cfg_rest, int* cfg_head) {
-  if (g_tile_override >= 0 || P.mode == FM_MODE_OZAKI_INT8) return;
+  if (P.tile >= 0 || P.mode == FM_MODE_OZAKI_INT8) return;
   const TileCand* c = find_cand(P.dtype, P.mode, cfg);
   if (!c) return;
   const int64_t slots = 256LL * c->per_cu;
@@ -591,6 +594,8 @@ fm_status build_dfs(fm_dtype dt, const Problem& p, const std::vector<Dims>& dims
     seq.dtype = dt;
     seq.mode = p.mode;
     seq.slices = p.slices;
+    seq.tile = p.tile;
+    seq.split_rem = p.split_rem;
     st = assemble(seq, child, child_base + child[0].ws_elems);
     if (st != FM_OK) return st;
     for (Launch& L : seq.launches) out.pre.push_back(std::move(L));
@@ -790,19 +795,19 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
   if (!leaves.empty()) {
     if (P.mode == FM_MODE_OZAKI_INT8) {
       Launch L;
-      fm_status st = ozaki_launch(P.slices, leaves, &wsb, L);
+      fm_status st = ozaki_launch(P.slices, P.tile, leaves, &wsb, L);
       if (st != FM_OK) return st;
       P.launches.push_back(std::move(L));
     } else {
-      const int cfg = pick_tile(P.dtype, P.mode, leaves);
+      const int cfg = pick_tile(P.dtype, P.mode, P.tile, leaves);
       if (fm_tile_planes(cfg) && P.mode != FM_MODE_SPLIT_BF16)
         return set_err(FM_ERR_INVALID_ARG, "tile config " + std::to_string(cfg) + " is for split_bf16 leaves");
       std::vector<FmGemmDesc> rest, strips;
       int cfg_rest = cfg, cfg_head = cfg, cfg_main = cfg, cfg_strips = cfg;
-      if (persistent_grid == 0 && g_split_remainder &&
+      if (persistent_grid == 0 && P.split_rem &&
           carve_full(P, cfg, leaves, strips, &cfg_main, &cfg_strips))
         cfg_rest = cfg_head = cfg_main;
-      if (persistent_grid == 0 && g_split_remainder)
+      if (persistent_grid == 0 && P.split_rem)
         split_remainder(P, cfg_main, leaves, rest, &cfg_rest, &cfg_head);
       auto leaf = [&](std::vector<FmGemmDesc> prods, int c) {
         int64_t kmax = 0;
@@ -841,8 +846,10 @@ Plan::~Plan() {
   if (dbuf) (void)hipFree(dbuf);
 }
 
-fm_status Plan::upload() {
-  std::vector<char> buf;
+// Descriptors and the role table, packed into one host buffer; each
+// launch records its offsets in it.
+void Plan::pack(std::vector<char>& buf) {
+  buf.clear();
   auto put = [&](const void* src, size_t bytes) {
     const size_t off = (size_t)align_up((int64_t)buf.size(), 256);
     buf.resize(off + bytes);
@@ -861,6 +868,28 @@ fm_status Plan::upload() {
       L.dev_off = put(L.ops.data(), L.ops.size() * sizeof(FmOp));
   }
   ext_off = put(ext.data(), ext.size() * sizeof(void*));
+}
+
+// Point every launch at its descriptors in a device copy of the packed buffer.
+void Plan::bind(void* dev) {
+  desc_base = dev;
+  for (Launch& L : launches) {
+    char* p = (char*)dev + L.dev_off;
+    if (L.kind == LK_GEMM)
+      L.gm.descs = reinterpret_cast<const FmGemmDesc*>(p);
+    else if (L.kind == LK_OZAKI)
+      L.oz.descs = reinterpret_cast<const FmOzakiDesc*>(p);
+    else if (L.kind == LK_SPLIT) {
+      L.sp.descs = reinterpret_cast<const FmSplitDesc*>(p);
+      L.sp.preps = reinterpret_cast<const FmSplitPrep*>((char*)dev + L.dev_off2);
+    } else
+      L.st.ops = reinterpret_cast<const FmOp*>(p);
+  }
+}
+
+fm_status Plan::upload() {
+  std::vector<char> buf;
+  pack(buf);
   if (buf.empty()) return FM_OK;
   hipError_t e = hipMalloc(&dbuf, buf.size());
   if (e != hipSuccess) {
@@ -869,18 +898,7 @@ fm_status Plan::upload() {
   }
   e = hipMemcpy(dbuf, buf.data(), buf.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) return hip_err(e, "hipMemcpy(plan)");
-  for (Launch& L : launches) {
-    char* p = (char*)dbuf + L.dev_off;
-    if (L.kind == LK_GEMM)
-      L.gm.descs = reinterpret_cast<const FmGemmDesc*>(p);
-    else if (L.kind == LK_OZAKI)
-      L.oz.descs = reinterpret_cast<const FmOzakiDesc*>(p);
-    else if (L.kind == LK_SPLIT) {
-      L.sp.descs = reinterpret_cast<const FmSplitDesc*>(p);
-      L.sp.preps = reinterpret_cast<const FmSplitPrep*>((char*)dbuf + L.dev_off2);
-    } else
-      L.st.ops = reinterpret_cast<const FmOp*>(p);
-  }
+  bind(dbuf);
   return FM_OK;
 }
 
@@ -932,7 +950,7 @@ fm_status Plan::run(const void* A, const void* B, void* C, void* ws, hipStream_t
   bases.p[FM_ROLE_B] = const_cast<void*>(B);
   bases.p[FM_ROLE_C] = C;
   bases.p[FM_ROLE_WS] = ws;
-  bases.ext = ext.empty() ? nullptr : reinterpret_cast<void* const*>((char*)dbuf + ext_off);
+  bases.ext = ext.empty() ? nullptr : reinterpret_cast<void* const*>((char*)desc_base + ext_off);
   bool aligned = true;
   for (int i = 0; i < 4; ++i) aligned = aligned && ((uintptr_t)bases.p[i] % 16 == 0);
   for (void* p : ext) aligned = aligned && ((uintptr_t)p % 16 == 0);

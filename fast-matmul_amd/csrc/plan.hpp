@@ -46,13 +46,19 @@ struct Plan {
   fm_dtype dtype = FM_F64;
   fm_mode mode = FM_MODE_PLAIN;
   int slices = 0;
+  int tile = -1;       // forced leaf tile configuration (-1: cost model)
+  int split_rem = 1;   // whole-tile carve-out and remainder launch
+  int dfs_levels = 0;  // depth-first top levels the plan runs
   std::vector<Launch> launches;
   std::vector<void*> ext;          // role table (roles >= 4) of batched plans
-  void* dbuf = nullptr;            // descriptors + role table on the device
+  void* dbuf = nullptr;            // descriptors + role table on the device (owned)
+  void* desc_base = nullptr;       // where the launches read them (dbuf, or a caller's copy)
   size_t ext_off = 0;
   size_t ws_bytes = 0;
   ~Plan();
-  fm_status upload();
+  void pack(std::vector<char>& buf);
+  void bind(void* dev);
+  fm_status upload();  // pack, copy into a new dbuf, bind
   // Launch every kernel on `s`; A/B/C/ws fill roles 0-3.
   fm_status run(const void* A, const void* B, void* C, void* ws, hipStream_t s) const;
 };
@@ -70,6 +76,7 @@ struct Problem {
   int dfs = 0;       // top levels run depth-first (one child sub-plan at a time)
   fm_mode mode = FM_MODE_PLAIN;  // leaf mode and digit count, for depth-first children
   int slices = 0;
+  int tile = -1, split_rem = 1;  // leaf launch options, for depth-first children
 };
 
 struct Dims {
@@ -101,16 +108,21 @@ fm_status build_dfs(fm_dtype dt, const Problem& p, const std::vector<Dims>& dims
 // 0 launches one work group per tile.
 fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int persistent_grid = 0);
 
-int pick_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods);
+int pick_tile(fm_dtype dt, fm_mode mode, int tile, const std::vector<FmGemmDesc>& prods);
 // move the products of a part-empty last round of tiles to `rest`
 void split_remainder(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods,
                      std::vector<FmGemmDesc>& rest, int* cfg_rest, int* cfg_head);
-extern int g_split_remainder;
-extern int g_dfs_levels;
-extern size_t g_ws_limit;
-extern int g_tile_override;
-extern int g_ozaki_slices;
-extern int g_stage_fusion;
+// process defaults of the per-call options (fm_set_* change them)
+extern fm_options g_defaults;
+fm_status check_options(const fm_options& o);
+fm_status check_args(fm_dtype dt, int m, int k, int n, const void* A, int64_t lda, const void* B,
+                     int64_t ldb, const void* C, int64_t ldc);
+fm_status check_mode(fm_dtype dt, fm_mode mode);
+// One multiply's plan (alg == nullptr: the classical product) under options
+// `o`, not uploaded.
+fm_status build_single(Plan& P, const FmAlgDesc* alg, fm_dtype dt, fm_mode mode, int64_t m,
+                       int64_t k, int64_t n, int64_t lda, int64_t ldb, int64_t ldc, int steps,
+                       double x, double alpha, double beta, const fm_options& o);
 
 // profiling (fm_profile_enable / fm_profile_read)
 extern bool g_prof_on;

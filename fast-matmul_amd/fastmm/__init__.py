@@ -26,7 +26,7 @@ from . import tables
 from ._lib import LIB, FastMMError, check  # noqa: F401
 
 __all__ = ["Matrix", "fastmatmul", "fastmatmul_scaled", "gemm", "MatMul", "RunAlgorithm", "alg", "algorithm_names",
-           "workspace_bytes", "release_caches"]
+           "workspace_bytes", "release_caches", "options", "Plan", "scaled_workspace_bytes"]
 
 _DT = {torch.float64: _lib.FM_F64, torch.float32: _lib.FM_F32}
 
@@ -76,42 +76,136 @@ def _stream_handle(t: torch.Tensor):
     return ctypes.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 
 
-def fastmatmul(alg_name: str, A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None,
-               num_steps: int = 1, x: float = 1e-8, alpha: float = 1.0, beta: float = 0.0,
-               mode: str = "plain", workspace: Optional[torch.Tensor] = None,
-               timed: bool = False):
-    """C := alpha * A @ B + beta * C with ``num_steps`` levels of ``alg_name``.
-
-    A, B, C are column-major GPU tensors of one dtype (float64 or float32).
-    Returns C, or (C, ms) when ``timed``."""
+def _operands(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor], beta: float):
+    """Check A (m x k), B (k x n) and C (m x n, allocated when None): 2-D
+    column-major views of one float64/float32 dtype on one GPU.  Every entry
+    point runs this before anything reaches the library, so a mismatched
+    call raises here instead of letting a kernel write past C."""
     A = _col_major(A, "A")
     B = _col_major(B, "B")
     m, k = A.shape
     k2, n = B.shape
     if k2 != k:
-        raise ValueError("inner dimensions differ: %d vs %d" % (k, k2))
-    if A.dtype != B.dtype or A.dtype not in _DT:
-        raise TypeError("A and B must share a float64/float32 dtype")
+        raise ValueError("inner dimensions differ: A is %d x %d, B is %d x %d" % (m, k, k2, n))
+    if A.dtype not in _DT:
+        raise TypeError("A must be float64 or float32, not %s" % A.dtype)
+    if B.dtype != A.dtype:
+        raise TypeError("B must have A's dtype %s, not %s" % (A.dtype, B.dtype))
     if not A.is_cuda or not B.is_cuda:
         raise ValueError("A and B must be GPU tensors")
+    if B.device != A.device:
+        raise ValueError("A and B must be on one device (%s vs %s)" % (A.device, B.device))
     if C is None:
-        C = empty(m, n, A.dtype, A.device)
         if beta != 0.0:
             raise ValueError("beta != 0 needs an input C")
+        C = empty(m, n, A.dtype, A.device)
     C = _col_major(C, "C")
-    if C.shape != (m, n) or C.dtype != A.dtype:
-        raise ValueError("C must be %d x %d of dtype %s" % (m, n, A.dtype))
-    ws_ptr, ws_bytes = None, 0
-    if workspace is not None:
-        ws_ptr, ws_bytes = ctypes.c_void_p(workspace.data_ptr()), workspace.numel() * workspace.element_size()
+    if tuple(C.shape) != (m, n):
+        raise ValueError("C must be %d x %d, not %d x %d" % (m, n, C.shape[0], C.shape[1]))
+    if C.dtype != A.dtype:
+        raise TypeError("C must have A's dtype %s, not %s" % (A.dtype, C.dtype))
+    if not C.is_cuda or C.device != A.device:
+        raise ValueError("C must be a GPU tensor on A's device %s" % A.device)
+    return A, B, C, m, k, n
+
+
+def options(**kw) -> "_lib.FmOptions":
+    """An fm_options block: the process defaults with the given fields
+    replaced (gemm_tile, dfs_levels, workspace_limit, split_remainder,
+    stage_fusion, ozaki_slices)."""
+    o = _lib.FmOptions()
+    LIB.fm_options_default(ctypes.byref(o))
+    names = {f[0] for f in _lib.FmOptions._fields_}
+    for key, val in kw.items():
+        if key not in names:
+            raise TypeError("unknown option %r" % key)
+        setattr(o, key, int(val))
+    return o
+
+
+def _options_ref(opts):
+    if opts is None:
+        return None
+    if isinstance(opts, dict):
+        opts = options(**opts)
+    return ctypes.byref(opts)
+
+
+def _workspace(workspace, A):
+    if workspace is None:
+        return None, 0
+    if not workspace.is_cuda or workspace.device != A.device:
+        raise ValueError("workspace must be a GPU tensor on A's device")
+    return ctypes.c_void_p(workspace.data_ptr()), workspace.numel() * workspace.element_size()
+
+
+def fastmatmul(alg_name: str, A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None,
+               num_steps: int = 1, x: float = 1e-8, alpha: float = 1.0, beta: float = 0.0,
+               mode: str = "plain", workspace: Optional[torch.Tensor] = None,
+               timed: bool = False, opts=None):
+    """C := alpha * A @ B + beta * C with ``num_steps`` levels of ``alg_name``.
+
+    A, B, C are column-major GPU tensors of one dtype (float64 or float32).
+    ``opts`` (a dict or ``options(...)``) overrides the process defaults for
+    this call only.  Returns C, or (C, ms) when ``timed``."""
+    A, B, C, m, k, n = _operands(A, B, C, beta)
+    ws_ptr, ws_bytes = _workspace(workspace, A)
     ms = ctypes.c_float(0.0)
-    st = LIB.fm_fastmatmul(alg_name.encode(), _DT[A.dtype], _lib.MODES[mode], m, k, n,
-                           ctypes.c_void_p(A.data_ptr()), _ld(A), ctypes.c_void_p(B.data_ptr()),
-                           _ld(B), ctypes.c_void_p(C.data_ptr()), _ld(C), int(num_steps),
-                           float(x), float(alpha), float(beta), _stream_handle(A), ws_ptr,
-                           ws_bytes, ctypes.byref(ms) if timed else None)
+    st = LIB.fm_fastmatmul_ex(alg_name.encode(), _DT[A.dtype], _lib.MODES[mode], m, k, n,
+                              ctypes.c_void_p(A.data_ptr()), _ld(A), ctypes.c_void_p(B.data_ptr()),
+                              _ld(B), ctypes.c_void_p(C.data_ptr()), _ld(C), int(num_steps),
+                              float(x), float(alpha), float(beta), _options_ref(opts),
+                              _stream_handle(A), ws_ptr, ws_bytes,
+                              ctypes.byref(ms) if timed else None)
     check(st)
     return (C, ms.value) if timed else C
+
+
+class Plan:
+    """One multiply's launch sequence, built once (fm_plan_create) and run
+    for any A, B, C of the planned shape and leading dimensions
+    (fm_plan_run).  ``alg`` None plans the classical product."""
+
+    def __init__(self, alg: Optional[str], m: int, k: int, n: int, num_steps: int = 1,
+                 dtype=torch.float64, mode: str = "plain", lda: Optional[int] = None,
+                 ldb: Optional[int] = None, ldc: Optional[int] = None, x: float = 1e-8,
+                 alpha: float = 1.0, beta: float = 0.0, opts=None):
+        if dtype not in _DT:
+            raise TypeError("dtype must be float64 or float32")
+        self.shape = (int(m), int(k), int(n))
+        self.dtype = dtype
+        self.lds = (int(lda or max(1, m)), int(ldb or max(1, k)), int(ldc or max(1, m)))
+        h = ctypes.c_void_p()
+        check(LIB.fm_plan_create(alg.encode() if alg else None, _DT[dtype], _lib.MODES[mode],
+                                 int(m), int(k), int(n), self.lds[0], self.lds[1], self.lds[2],
+                                 int(num_steps), float(x), float(alpha), float(beta),
+                                 _options_ref(opts), ctypes.byref(h)))
+        self._h = h
+
+    def workspace_bytes(self) -> int:
+        out = ctypes.c_size_t(0)
+        check(LIB.fm_plan_workspace(self._h, ctypes.byref(out)))
+        return out.value
+
+    def run(self, A: torch.Tensor, B: torch.Tensor, C: torch.Tensor,
+            workspace: Optional[torch.Tensor] = None, timed: bool = False):
+        A, B, C, m, k, n = _operands(A, B, C, 0.0)
+        if (m, k, n) != self.shape or A.dtype != self.dtype:
+            raise ValueError("plan is for %s %s, not %s %s" % (self.shape, self.dtype, (m, k, n), A.dtype))
+        if (_ld(A), _ld(B), _ld(C)) != self.lds:
+            raise ValueError("plan is for leading dimensions %s, not %s" % (self.lds, (_ld(A), _ld(B), _ld(C))))
+        ws_ptr, ws_bytes = _workspace(workspace, A)
+        ms = ctypes.c_float(0.0)
+        check(LIB.fm_plan_run(self._h, ctypes.c_void_p(A.data_ptr()), ctypes.c_void_p(B.data_ptr()),
+                              ctypes.c_void_p(C.data_ptr()), _stream_handle(A), ws_ptr, ws_bytes,
+                              ctypes.byref(ms) if timed else None))
+        return (C, ms.value) if timed else C
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and LIB is not None:
+            LIB.fm_plan_destroy(h)
+            self._h = None
 
 
 def fastmatmul_scaled(alg_name: str, A: torch.Tensor, B: torch.Tensor,
@@ -122,21 +216,7 @@ def fastmatmul_scaled(alg_name: str, A: torch.Tensor, B: torch.Tensor,
     """fastmatmul on diagonally scaled copies of A and B, scaled back
     (scaling/scaling.hpp:105-196): C := alpha diag(r) (A' B') diag(s) + beta C.
     ``scaling`` is one of outside_inside, inside_outside, inside, outside."""
-    A = _col_major(A, "A")
-    B = _col_major(B, "B")
-    m, k = A.shape
-    k2, n = B.shape
-    if k2 != k:
-        raise ValueError("inner dimensions differ: %d vs %d" % (k, k2))
-    if A.dtype != B.dtype or A.dtype not in _DT:
-        raise TypeError("A and B must share a float64/float32 dtype")
-    if C is None:
-        C = empty(m, n, A.dtype, A.device)
-        if beta != 0.0:
-            raise ValueError("beta != 0 needs an input C")
-    C = _col_major(C, "C")
-    if C.shape != (m, n) or C.dtype != A.dtype:
-        raise ValueError("C must be %d x %d of dtype %s" % (m, n, A.dtype))
+    A, B, C, m, k, n = _operands(A, B, C, beta)
     if scaling not in _lib.SCALINGS:
         raise ValueError("scaling must be one of %s" % sorted(_lib.SCALINGS))
     ms = ctypes.c_float(0.0)
@@ -153,13 +233,7 @@ def fastmatmul_scaled(alg_name: str, A: torch.Tensor, B: torch.Tensor,
 def gemm(A: torch.Tensor, B: torch.Tensor, C: Optional[torch.Tensor] = None, alpha: float = 1.0,
          beta: float = 0.0, mode: str = "plain") -> torch.Tensor:
     """Classical C := alpha * A @ B + beta * C on the library's leaf kernel."""
-    A = _col_major(A, "A")
-    B = _col_major(B, "B")
-    m, k = A.shape
-    n = B.shape[1]
-    if C is None:
-        C = empty(m, n, A.dtype, A.device)
-    C = _col_major(C, "C")
+    A, B, C, m, k, n = _operands(A, B, C, beta)
     st = LIB.fm_gemm(_DT[A.dtype], _lib.MODES[mode], m, k, n, ctypes.c_void_p(A.data_ptr()), _ld(A),
                      ctypes.c_void_p(B.data_ptr()), _ld(B), ctypes.c_void_p(C.data_ptr()), _ld(C),
                      float(alpha), float(beta), _stream_handle(A))
@@ -175,8 +249,25 @@ def workspace_bytes(alg_name: str, m: int, k: int, n: int, num_steps: int,
     return out.value
 
 
+def scaled_workspace_bytes(alg_name: str, m: int, k: int, n: int, num_steps: int,
+                           beta_nonzero: bool = False, dtype=torch.float64,
+                           mode: str = "plain") -> int:
+    """Workspace fastmatmul_scaled needs (fm_scaled_workspace_query)."""
+    out = ctypes.c_size_t(0)
+    check(LIB.fm_scaled_workspace_query(alg_name.encode(), _DT[dtype], _lib.MODES[mode], m, k, n,
+                                        num_steps, 1 if beta_nonzero else 0, ctypes.byref(out)))
+    return out.value
+
+
 def release_caches():
     LIB.fm_release_caches()
+
+
+def release_workspace(stream=None):
+    """Free the library's cached workspace of ``stream`` (default: the
+    current stream of the current device)."""
+    s = stream if stream is not None else torch.cuda.current_stream()
+    LIB.fm_release_workspace(ctypes.c_void_p(s.cuda_stream))
 
 
 def set_gemm_tile(cfg: int = -1):
@@ -196,7 +287,7 @@ def plan_info(alg_name: str, m: int, k: int, n: int, num_steps: int = 1,
             "two_level_stages": info.two_level_stages,
             "leaf": [{"cfg": info.leaf_cfg[i], "products": info.leaf_products[i],
                       "tiles": info.leaf_tiles[i]} for i in range(min(2, info.launches[3]))],
-            "workspace_bytes": info.workspace_bytes}
+            "workspace_bytes": info.workspace_bytes, "dfs_levels": info.dfs_levels}
 
 
 def set_schedule(dfs_levels: int = 0, workspace_limit: int = 0, split_remainder: bool = True):
@@ -244,11 +335,18 @@ class Batch:
         self._keep = []
         arr = (_lib.FmProblem * len(problems))()
         dtype = None
+        device = None
         for i, p in enumerate(problems):
-            A, B, C = _col_major(p["A"], "A"), _col_major(p["B"], "B"), _col_major(p["C"], "C")
+            try:
+                A, B, C, _, _, _ = _operands(p["A"], p["B"], p["C"], float(p.get("beta", 0.0)))
+            except (ValueError, TypeError) as e:
+                raise type(e)("problem %d: %s" % (i, e)) from None
             dtype = dtype or A.dtype
-            if A.dtype != dtype or B.dtype != dtype or C.dtype != dtype:
-                raise TypeError("all matrices of a batch share one dtype")
+            device = device or A.device
+            if A.dtype != dtype:
+                raise TypeError("problem %d: all matrices of a batch share one dtype" % i)
+            if A.device != device:
+                raise ValueError("problem %d: all matrices of a batch share one device" % i)
             self._keep += [A, B, C]
             alg = p.get("alg")
             arr[i] = _lib.FmProblem(alg.encode() if alg else None, int(p.get("num_steps", -1)),

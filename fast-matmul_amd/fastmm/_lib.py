@@ -32,6 +32,8 @@ EXPORTS = (
     "fm_set_ozaki_slices", "fm_batch_create", "fm_batch_run", "fm_batch_choice",
     "fm_batch_workspace_bytes", "fm_batch_destroy", "fm_autotune_estimate",
     "fm_set_stage_fusion", "fm_set_schedule", "fm_plan_query", "fm_fastmatmul_scaled",
+    "fm_options_default", "fm_fastmatmul_ex", "fm_plan_create", "fm_plan_run",
+    "fm_plan_workspace", "fm_plan_destroy", "fm_scaled_workspace_query", "fm_release_workspace",
 )
 
 # scaling rules of scaling/scaling.hpp:17-22
@@ -44,7 +46,14 @@ class FmPlanInfo(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_int * 5), ("stage_bytes", ctypes.c_double * 5),
                 ("two_level_stages", ctypes.c_int),
                 ("leaf_cfg", ctypes.c_int * 2), ("leaf_products", ctypes.c_int * 2),
-                ("leaf_tiles", ctypes.c_int * 2), ("workspace_bytes", ctypes.c_size_t)]
+                ("leaf_tiles", ctypes.c_int * 2), ("workspace_bytes", ctypes.c_size_t),
+                ("dfs_levels", ctypes.c_int)]
+
+
+class FmOptions(ctypes.Structure):
+    _fields_ = [("gemm_tile", ctypes.c_int), ("dfs_levels", ctypes.c_int),
+                ("workspace_limit", ctypes.c_size_t), ("split_remainder", ctypes.c_int),
+                ("stage_fusion", ctypes.c_int), ("ozaki_slices", ctypes.c_int)]
 
 
 class FmProfile(ctypes.Structure):
@@ -77,6 +86,29 @@ def _load():
                                   vp, i64, vp, i64, vp, i64, c_int, c_double, c_double,
                                   c_double, vp, vp, ctypes.c_size_t,
                                   ctypes.POINTER(ctypes.c_float)]
+    lib.fm_fastmatmul_ex.restype = c_int
+    lib.fm_fastmatmul_ex.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int,
+                                     vp, i64, vp, i64, vp, i64, c_int, c_double, c_double,
+                                     c_double, ctypes.POINTER(FmOptions), vp, vp, ctypes.c_size_t,
+                                     ctypes.POINTER(ctypes.c_float)]
+    lib.fm_options_default.restype = None
+    lib.fm_options_default.argtypes = [ctypes.POINTER(FmOptions)]
+    lib.fm_plan_create.restype = c_int
+    lib.fm_plan_create.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int, i64, i64,
+                                   i64, c_int, c_double, c_double, c_double,
+                                   ctypes.POINTER(FmOptions), ctypes.POINTER(vp)]
+    lib.fm_plan_run.restype = c_int
+    lib.fm_plan_run.argtypes = [vp, vp, vp, vp, vp, vp, ctypes.c_size_t,
+                                ctypes.POINTER(ctypes.c_float)]
+    lib.fm_plan_workspace.restype = c_int
+    lib.fm_plan_workspace.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    lib.fm_plan_destroy.restype = None
+    lib.fm_plan_destroy.argtypes = [vp]
+    lib.fm_scaled_workspace_query.restype = c_int
+    lib.fm_scaled_workspace_query.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int,
+                                              c_int, c_int, ctypes.POINTER(ctypes.c_size_t)]
+    lib.fm_release_workspace.restype = None
+    lib.fm_release_workspace.argtypes = [vp]
     lib.fm_fastmatmul_scaled.restype = c_int
     lib.fm_fastmatmul_scaled.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int,
                                          vp, i64, vp, i64, vp, i64, c_int, c_double, c_double,

@@ -78,6 +78,60 @@ fm_status fm_fastmatmul(const char* alg, fm_dtype dtype, fm_mode mode, int m, in
 fm_status fm_workspace_query(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k,
                              int n, int num_steps, size_t* bytes);
 
+/* ---- per-call options and reusable plans ----------------------------------
+ * The reference fixes its schedule at compile time (_PARALLEL_, util/
+ * par_util.hpp:17-95) and its leaf kernel by the linked BLAS.  Here every
+ * choice a plan embeds is an explicit option, given per call or per plan:
+ *   gemm_tile        -1: chosen per leaf launch by the tile cost model;
+ *                    0 .. 31 forces an FmTileCfg (see fm_set_gemm_tile);
+ *   dfs_levels       top recursion levels run depth-first (0: breadth-first;
+ *                    -1: as few as keep the workspace under workspace_limit,
+ *                    or, with no limit, under a quarter of the device's
+ *                    free memory at plan time);
+ *   workspace_limit  bytes (0: none): depth-first levels are added from the
+ *                    top until the plan fits (FM_ERR_WORKSPACE otherwise);
+ *   split_remainder  1: whole-tile carve-out and the remainder launch;
+ *   stage_fusion     1: two-level S/T/W stages where the shape allows;
+ *   ozaki_slices     6, 7 or 8 int8 digits (FM_MODE_OZAKI_INT8).
+ * A NULL options pointer means the process defaults, which the fm_set_*
+ * calls below change; fm_options_default reads them.  Results are
+ * bit-identical under every schedule and tile choice. */
+typedef struct {
+  int gemm_tile;
+  int dfs_levels;
+  size_t workspace_limit;
+  int split_remainder;
+  int stage_fusion;
+  int ozaki_slices;
+} fm_options;
+
+void fm_options_default(fm_options* out);
+
+/* fm_fastmatmul with explicit options (opts may be NULL). */
+fm_status fm_fastmatmul_ex(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k, int n,
+                           const void* A, int64_t lda, const void* B, int64_t ldb, void* C,
+                           int64_t ldc, int num_steps, double x, double alpha, double beta,
+                           const fm_options* opts, void* stream, void* workspace,
+                           size_t workspace_bytes, float* ms_out);
+
+/* A plan: one multiply's launch sequence, built and uploaded once, run for
+ * any A, B, C of the planned shape and leading dimensions.  fm_plan_run
+ * issues kernel launches only (no allocation when a workspace is given, no
+ * memset), so a caller can capture it into a hipGraph.  No reference
+ * counterpart: the reference rebuilds its recursion on every call
+ * (algorithms/strassen.hpp:662-722).  alg == NULL plans the classical
+ * product (beta honoured). */
+typedef struct fm_plan_s* fm_plan;
+fm_status fm_plan_create(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k, int n,
+                         int64_t lda, int64_t ldb, int64_t ldc, int num_steps, double x,
+                         double alpha, double beta, const fm_options* opts, fm_plan* out);
+/* workspace == NULL: the library's arena for `stream` (which may allocate
+ * on first use: do one run before capturing a graph, or pass a buffer). */
+fm_status fm_plan_run(fm_plan plan, const void* A, const void* B, void* C, void* stream,
+                      void* workspace, size_t workspace_bytes, float* ms_out);
+fm_status fm_plan_workspace(fm_plan plan, size_t* bytes);
+void fm_plan_destroy(fm_plan plan);
+
 /* Classical C := alpha * A * B + beta * C on the same leaf kernels
  * (MatMul, linalg/linalg.hpp:261-269), with the same leaf modes. */
 fm_status fm_gemm(fm_dtype dtype, fm_mode mode, int m, int k, int n, const void* A,
@@ -157,14 +211,15 @@ typedef struct {
 void fm_profile_enable(int on);
 fm_status fm_profile_read(fm_profile* out);
 
-/* Digit count S of FM_MODE_OZAKI_INT8 leaves (6, 7 or 8; default 8): each
+/* Process default of fm_options.ozaki_slices.
+ * Digit count S of FM_MODE_OZAKI_INT8 leaves (6, 7 or 8; default 8): each
  * FP64 operand row/column becomes S int8 digits of 7 bits under a
  * power-of-two scale, and S(S+1)/2 int8 products are summed. */
 fm_status fm_set_ozaki_slices(int slices);
 
-/* Tuning override: force leaf-GEMM tile configuration `cfg` (0 .. 31, see
- * FmTileCfg in the sources) for every plan built afterwards; -1 restores the
- * automatic choice.  Cached plans are dropped.  Configurations 24 .. 29 and 31 run
+/* Process default of fm_options.gemm_tile: force leaf-GEMM tile configuration
+ * `cfg` (0 .. 31, see FmTileCfg in the sources) for calls that pass no
+ * options; -1 restores the automatic choice.  Configurations 24 .. 29 and 31 run
  * split_bf16 leaves on prepared slice planes and are refused in other modes
  * (FM_ERR_INVALID_ARG when a plan is built); in ozaki_int8 mode, 1 (64 x 64)
  * selects the 16x16x64 digit kernel and any other value the default one. */
@@ -187,6 +242,11 @@ fm_status fm_fastmatmul_scaled(const char* alg, fm_dtype dtype, fm_mode mode, in
                                int scaling_type, int scaling_steps, void* stream, void* workspace,
                                size_t workspace_bytes, float* ms_out);
 
+/* Workspace bytes fm_fastmatmul_scaled needs (its plan's, the copies of A and
+ * B, the product when beta != 0, the scaling vectors). */
+fm_status fm_scaled_workspace_query(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k,
+                                    int n, int num_steps, int beta_nonzero, size_t* bytes);
+
 /* Shape of the plan fm_fastmatmul would run (built on the host, nothing
  * launched; contiguous operands): launches per fm_stage_kind, algorithmic
  * HBM bytes per fm_stage_kind (every source element read once, every output
@@ -201,18 +261,22 @@ typedef struct fm_plan_info {
   int leaf_products[2];
   int leaf_tiles[2];
   size_t workspace_bytes;
+  int dfs_levels; /* depth-first top levels the plan runs */
 } fm_plan_info;
 fm_status fm_plan_query(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k, int n,
                         int num_steps, fm_plan_info* info);
 
-/* Recursion schedule, the run-time form of the reference's compile-time
- * _PARALLEL_ choice of DFS, BFS or HYBRID (util/par_util.hpp:17-95,
- * algorithms/strassen.hpp:404-463); plans built afterwards follow it.
+/* Process defaults of fm_options' schedule fields: the run-time form of the
+ * reference's compile-time _PARALLEL_ choice of DFS, BFS or HYBRID
+ * (util/par_util.hpp:17-95, algorithms/strassen.hpp:404-463); calls that
+ * pass no options follow them.
  *  - dfs_levels = 0 (default): breadth-first, every level one launch per
  *    stage over all of its nodes and one grouped launch of all leaves.
  *  - dfs_levels = d > 0: the top d levels run depth-first, one child at a
  *    time (its whole breadth-first sub-plan), the children reusing one
  *    workspace region: less workspace, smaller leaf launches.
+ *  - dfs_levels = -1: the fewest depth-first levels that keep the workspace
+ *    under workspace_limit (or a quarter of the free device memory).
  *  - workspace_limit > 0: depth-first levels are added from the top until
  *    the plan's workspace fits the limit (FM_ERR_WORKSPACE if even a fully
  *    depth-first plan does not).
@@ -223,7 +287,8 @@ fm_status fm_plan_query(const char* alg, fm_dtype dtype, fm_mode mode, int m, in
  * Results are bit-identical under every schedule. */
 fm_status fm_set_schedule(int dfs_levels, size_t workspace_limit, int split_remainder);
 
-/* Two-level S/T stages (default on): where two consecutive recursion levels
+/* Process default of fm_options.stage_fusion.
+ * Two-level S/T stages (default on): where two consecutive recursion levels
  * have no peeling between them, one launch forms the deeper level's S (T)
  * operands straight from the upper level's, without storing the middle
  * level.  Results are bit-identical either way; off trades speed for the
@@ -232,8 +297,12 @@ fm_status fm_set_schedule(int dfs_levels, size_t workspace_limit, int split_rema
  * (algorithms/strassen.hpp:9-252). */
 void fm_set_stage_fusion(int on);
 
-/* Drop cached plans and the library-owned workspace arena. */
+/* Drop cached plans and the library-owned workspace arenas.  The arenas (one
+ * per device and stream) are bounded together by half the device's memory;
+ * fm_release_workspace frees the one of `stream` on the current device
+ * (after waiting for the device). */
 void fm_release_caches(void);
+void fm_release_workspace(void* stream);
 
 #ifdef __cplusplus
 }

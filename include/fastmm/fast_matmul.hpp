@@ -119,13 +119,39 @@ inline void hcheck(hipError_t e) {
   if (e != hipSuccess) throw std::runtime_error(std::string("fastmm: ") + hipGetErrorString(e));
 }
 
-// Device copy of a host view (packed, ld = m).
+// Device staging buffers, one per operand slot, kept per thread and grown on
+// demand: calls in a loop reuse them instead of allocating per call.  They
+// live until the thread ends (the HIP runtime reclaims device memory at
+// process exit; they are not freed from a thread-exit destructor, which may
+// run after the runtime has shut down).
+struct StagingSlot {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+inline void* staging(int slot, size_t bytes) {
+  thread_local StagingSlot slots[3];
+  StagingSlot& s = slots[slot];
+  if (s.bytes < bytes) {
+    if (s.p) {
+      hcheck(hipDeviceSynchronize());
+      (void)hipFree(s.p);
+      s.p = nullptr;
+      s.bytes = 0;
+    }
+    hcheck(hipMalloc(&s.p, bytes));
+    s.bytes = bytes;
+  }
+  return s.p;
+}
+
+// Device copy of a host view (packed, ld = m) in staging slot `slot`.
 template <typename Scalar>
 struct DevMat {
   Scalar* p = nullptr;
   int m, n;
-  explicit DevMat(const Matrix<Scalar>& h, bool upload) : m(h.m()), n(h.n()) {
-    hcheck(hipMalloc(&p, sizeof(Scalar) * (size_t)(m > 0 ? m : 1) * (n > 0 ? n : 1)));
+  DevMat(int slot, const Matrix<Scalar>& h, bool upload) : m(h.m()), n(h.n()) {
+    p = static_cast<Scalar*>(
+        staging(slot, sizeof(Scalar) * (size_t)(m > 0 ? m : 1) * (n > 0 ? n : 1)));
     if (upload && m > 0 && n > 0)
       hcheck(hipMemcpy2D(p, sizeof(Scalar) * m, h.data(), sizeof(Scalar) * h.stride(),
                          sizeof(Scalar) * m, n, hipMemcpyHostToDevice));
@@ -135,11 +161,6 @@ struct DevMat {
       hcheck(hipMemcpy2D(h.data(), sizeof(Scalar) * h.stride(), p, sizeof(Scalar) * m,
                          sizeof(Scalar) * m, n, hipMemcpyDeviceToHost));
   }
-  ~DevMat() {
-    if (p) (void)hipFree(p);
-  }
-  DevMat(const DevMat&) = delete;
-  DevMat& operator=(const DevMat&) = delete;
 };
 
 template <typename Scalar>
@@ -150,12 +171,17 @@ constexpr fm_dtype dtype_of() {
 }
 
 template <typename Scalar>
-double run(const char* alg, Matrix<Scalar>& A, Matrix<Scalar>& B, Matrix<Scalar>& C,
-           int num_steps, double x, Scalar alpha, Scalar beta) {
+void check_dims(const Matrix<Scalar>& A, const Matrix<Scalar>& B, const Matrix<Scalar>& C) {
   if (A.n() != B.m() || A.m() != C.m() || B.n() != C.n())
     throw std::invalid_argument("fastmm: inconsistent matrix dimensions");
+}
+
+template <typename Scalar>
+double run(const char* alg, Matrix<Scalar>& A, Matrix<Scalar>& B, Matrix<Scalar>& C,
+           int num_steps, double x, Scalar alpha, Scalar beta) {
+  check_dims(A, B, C);
   const Scalar eff = alpha * B.multiplier() * C.multiplier();
-  DevMat<Scalar> dA(A, true), dB(B, true), dC(C, beta != Scalar(0));
+  DevMat<Scalar> dA(0, A, true), dB(1, B, true), dC(2, C, beta != Scalar(0));
   float ms = 0.f;
   check(fm_fastmatmul(alg, dtype_of<Scalar>(), FM_MODE_PLAIN, A.m(), A.n(), B.n(), dA.p,
                       dA.m > 0 ? dA.m : 1, dB.p, dB.m > 0 ? dB.m : 1, dC.p,
@@ -170,7 +196,8 @@ double run(const char* alg, Matrix<Scalar>& A, Matrix<Scalar>& B, Matrix<Scalar>
 // C <- C.multiplier() * A B + beta C (linalg/linalg.hpp:261-269).
 template <typename Scalar>
 void MatMul(Matrix<Scalar>& A, Matrix<Scalar>& B, Matrix<Scalar>& C, Scalar beta = Scalar(0)) {
-  detail::DevMat<Scalar> dA(A, true), dB(B, true), dC(C, beta != Scalar(0));
+  detail::check_dims(A, B, C);
+  detail::DevMat<Scalar> dA(0, A, true), dB(1, B, true), dC(2, C, beta != Scalar(0));
   detail::check(fm_gemm(detail::dtype_of<Scalar>(), FM_MODE_PLAIN, A.m(), A.n(), B.n(), dA.p,
                         dA.m > 0 ? dA.m : 1, dB.p, dB.m > 0 ? dB.m : 1, dC.p,
                         dC.m > 0 ? dC.m : 1, (double)C.multiplier(), (double)beta, nullptr));

@@ -127,3 +127,43 @@ def test_autotune_cost_model():
     assert fastmm.autotune_estimate("strassen", 8192, 8192, 8192, 1, f32) < \
         fastmm.autotune_estimate(None, 8192, 8192, 8192, 0, f32)
     assert math.isinf(fastmm.autotune_estimate("strassen", 3, 3, 3, 2, f32))
+
+
+def test_options_defaults_and_rejects():
+    """fm_options_default reads the process defaults; bad options are refused
+    before anything is planned."""
+    import fastmm
+    from fastmm import _lib
+    o = fastmm.options()
+    assert (o.gemm_tile, o.dfs_levels, o.workspace_limit, o.split_remainder, o.stage_fusion,
+            o.ozaki_slices) == (-1, 0, 0, 1, 1, 8)
+    with pytest.raises(TypeError):
+        fastmm.options(no_such_field=1)
+    L = _lib.LIB
+    nul = ctypes.c_void_p(0)
+    for bad in (dict(gemm_tile=99), dict(dfs_levels=-2), dict(ozaki_slices=5)):
+        ob = fastmm.options(**bad)
+        st = L.fm_fastmatmul_ex(b"strassen", 0, 0, 8, 8, 8, ctypes.c_void_p(16), 8,
+                                ctypes.c_void_p(16), 8, ctypes.c_void_p(16), 8, 1, 1e-8, 1.0, 0.0,
+                                ctypes.byref(ob), None, None, 0, None)
+        assert st == _lib.FM_ERR_INVALID_ARG, bad
+    h = ctypes.c_void_p()
+    assert L.fm_plan_create(b"nope", 0, 0, 8, 8, 8, 8, 8, 8, 1, 1e-8, 1.0, 0.0, None,
+                            ctypes.byref(h)) == _lib.FM_ERR_UNKNOWN_ALG
+    assert L.fm_plan_create(b"strassen", 0, 0, 8, 8, 8, 4, 8, 8, 1, 1e-8, 1.0, 0.0, None,
+                            ctypes.byref(h)) == _lib.FM_ERR_INVALID_ARG
+    assert not h.value
+    assert L.fm_plan_run(None, nul, nul, nul, None, None, 0, None) == _lib.FM_ERR_INVALID_ARG
+
+
+def test_scaled_workspace_query():
+    """The scaled call's workspace: its plan's, the copies of A and B, the
+    product when beta != 0 and the scaling vectors, each 256-byte aligned."""
+    import fastmm
+    al = lambda v: (v + 255) // 256 * 256
+    m, k, n = 300, 200, 250
+    plan = fastmm.workspace_bytes("strassen", m, k, n, 2)
+    L = max(m, k, n)
+    base = al(plan) + al(8 * m * k) + al(8 * k * n) + al(8 * m) + al(8 * n) + 2 * al(8 * L) + 2 * al(8 * L)
+    assert fastmm.scaled_workspace_bytes("strassen", m, k, n, 2) == base
+    assert fastmm.scaled_workspace_bytes("strassen", m, k, n, 2, beta_nonzero=True) == base + al(8 * m * n)

@@ -77,12 +77,17 @@ def test_workspace_limit_selects_depth_first():
     fastmm.set_schedule(0, bfs)
     assert fastmm.plan_info("strassen", 256, 256, 256, 2)["workspace_bytes"] == bfs
     fastmm.set_schedule(0, bfs - 1)
-    assert fastmm.plan_info("strassen", 256, 256, 256, 2)["workspace_bytes"] == dfs
+    info = fastmm.plan_info("strassen", 256, 256, 256, 2)
+    assert info["workspace_bytes"] == dfs and info["dfs_levels"] == 1
+    # the automatic cut-off under the same limit chooses the same plan
+    fastmm.set_schedule(-1, bfs - 1)
+    info = fastmm.plan_info("strassen", 256, 256, 256, 2)
+    assert info["workspace_bytes"] == dfs and info["dfs_levels"] == 1
     fastmm.set_schedule(0, 1000)
     with pytest.raises(fastmm.FastMMError):
         fastmm.plan_info("strassen", 256, 256, 256, 2)
     with pytest.raises(ValueError):
-        fastmm.set_schedule(-1)
+        fastmm.set_schedule(-2)
 
 
 def test_remainder_split_modes():
