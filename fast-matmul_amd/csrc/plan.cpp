@@ -802,6 +802,13 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
       const int cfg = pick_tile(P.dtype, P.mode, P.tile, leaves);
       if (fm_tile_planes(cfg) && P.mode != FM_MODE_SPLIT_BF16)
         return set_err(FM_ERR_INVALID_ARG, "tile config " + std::to_string(cfg) + " is for split_bf16 leaves");
+      // split_bf16 leaves off the slice planes run the in-loop kernel, which
+      // has the 128 x 128, 64 x 64, 128 x 64, 64 x 128 and 128 x 128 (8-wave) tiles
+      if (P.mode == FM_MODE_SPLIT_BF16 && !fm_tile_planes(cfg) && cfg != FM_TILE_128x128 &&
+          cfg != FM_TILE_64x64 && cfg != FM_TILE_128x64 && cfg != FM_TILE_64x128 &&
+          cfg != FM_TILE_128x128_W8)
+        return set_err(FM_ERR_INVALID_ARG, "tile config " + std::to_string(cfg) +
+                                               " has no split_bf16 kernel");
       std::vector<FmGemmDesc> rest, strips;
       int cfg_rest = cfg, cfg_head = cfg, cfg_main = cfg, cfg_strips = cfg;
       if (persistent_grid == 0 && P.split_rem &&

@@ -5,6 +5,7 @@
 // the program checks bit equality and exits non-zero on any difference.
 #include <cstdio>
 #include <cstdlib>
+#include <stdexcept>
 
 #include "fastmm/fast_matmul.hpp"
 
@@ -51,5 +52,27 @@ int main() {
   for (int j = 0; j < 100; ++j)
     for (int i = 0; i < 150; ++i) bad += C1(i, j) != C2(i, j);
   std::printf("views: mismatches so far %d\n", bad);
+  // inconsistent dimensions are refused before anything is staged or run
+  bool threw = false;
+  Matrix<double> Cw(150, 99);
+  try {
+    MatMul(Ab, Bs, Cw);
+  } catch (const std::invalid_argument&) {
+    threw = true;
+  }
+  if (!threw) {
+    std::printf("MatMul accepted a C of the wrong width\n");
+    ++bad;
+  }
+  threw = false;
+  try {
+    strassen::FastMatmul(Ab, Bs, Cw, 1);
+  } catch (const std::invalid_argument&) {
+    threw = true;
+  }
+  if (!threw) {
+    std::printf("FastMatmul accepted a C of the wrong width\n");
+    ++bad;
+  }
   return bad ? 1 : 0;
 }

@@ -109,4 +109,11 @@ def golden_cases():
                             ("smirnov333_23_139", 2, (729, 729, 729))]:
         seed += 1
         cases.append(("u11_%s_%d_%d" % (alg, steps, shp[0]), alg) + shp + (steps, 1.0, 0.0, "u11", "f64", seed))
+    # the tall-skinny grey424_26_206 workload (131072 x 2400 x 2400) at an
+    # eighth of its row and column sizes: the same aspect, leaves of
+    # 4096 x 600 x 300 (one level) and 1024 x 300 x 75 (two)
+    for steps in (1, 2):
+        seed += 1
+        cases.append(("ts_grey424_26_206_%d_16384x1200x1200" % steps, "grey424_26_206", 16384, 1200,
+                      1200, steps, 1.0, 0.0, "u11", "f64", seed))
     return cases

@@ -29,7 +29,7 @@ OPTS = [
     dict(stage_fusion=0),
     dict(gemm_tile=1),                       # 64 x 64 leaf tiles
     dict(gemm_tile=12, stage_fusion=0),      # 128 x 64, 8 waves
-    dict(dfs_levels=-1, workspace_limit=1),  # as many depth-first levels as exist
+    dict(dfs_levels=-1, workspace_limit=16 << 20),  # automatic cut-off: one depth-first level
 ]
 
 

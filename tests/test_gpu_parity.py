@@ -221,6 +221,33 @@ def test_bench_workload_full_size(fm, oracle_mod):
     assert np.all(np.abs(got - q) <= 64 * 2 ** -53 * absab)
 
 
+@pytest.mark.parametrize("steps", [1, 2])
+def test_tall_skinny_workload_full_size(fm, oracle_mod, steps):
+    """grey424_26_206 on the tall-skinny workload, 131072 x 2400 x 2400 FP64,
+    uniform(-1,1) (benchmarks/matmul_benchmarks.cpp:275-326,
+    data/all_sequential_algs/tssquare.2400): leaves of 32768 x 1200 x 600
+    (one level) or 8192 x 600 x 150 (two), checked at full size through
+    sampled quad-precision entries and C x = A (B x)."""
+    m, k = 131072, 2400
+    g = torch.Generator(device=DEV)
+    g.manual_seed(0xFA57 + 2)
+    A = fm.col_major(torch.rand((m, k), generator=g, device=DEV, dtype=torch.float64) * 2 - 1)
+    B = fm.col_major(torch.rand((k, k), generator=g, device=DEV, dtype=torch.float64) * 2 - 1)
+    C = fm.fastmatmul("grey424_26_206", A, B, None, steps)
+    x = torch.rand((k, 1), generator=g, device=DEV, dtype=torch.float64)
+    lhs = C @ x
+    rhs = A @ (B @ x)
+    assert float((lhs - rhs).abs().max()) <= 1e-10 * k
+    rng = np.random.default_rng(2 + steps)
+    rows = rng.integers(0, m, 96)
+    cols = rng.integers(0, k, 96)
+    Ah, Bh = _host(A), _host(B)
+    q, absab = oracle_mod.quad_entries(Ah, Bh, rows, cols)
+    got = _host(C)[rows, cols]
+    # componentwise: |C - AB| <= c * u * (|A||B|)_ij with c well below K
+    assert np.all(np.abs(got - q) <= 64 * 2 ** -53 * absab)
+
+
 def test_cpp_header_program(tmp_path):
     """The C++ drop-in header, run on the GPU: exact products through
     strassen/grey424_26_206/smirnov333_23_139/grey423_20_144, views included."""
