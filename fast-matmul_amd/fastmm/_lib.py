@@ -34,7 +34,11 @@ EXPORTS = (
     "fm_set_stage_fusion", "fm_set_schedule", "fm_plan_query", "fm_fastmatmul_scaled",
     "fm_options_default", "fm_fastmatmul_ex", "fm_plan_create", "fm_plan_run",
     "fm_plan_workspace", "fm_plan_destroy", "fm_scaled_workspace_query", "fm_release_workspace",
+    "fm_fastmatmul_noalloc",
 )
+
+# the fixed scratch array of fm_fastmatmul_noalloc (FM_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 256 << 20
 
 # scaling rules of scaling/scaling.hpp:17-22
 SCALINGS = {"outside_inside": 0, "inside_outside": 1, "inside": 2, "outside": 3}
@@ -91,6 +95,10 @@ def _load():
                                      vp, i64, vp, i64, vp, i64, c_int, c_double, c_double,
                                      c_double, ctypes.POINTER(FmOptions), vp, vp, ctypes.c_size_t,
                                      ctypes.POINTER(ctypes.c_float)]
+    lib.fm_fastmatmul_noalloc.restype = c_int
+    lib.fm_fastmatmul_noalloc.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int,
+                                          vp, i64, vp, i64, vp, i64, c_int, c_double, c_double,
+                                          c_double, vp, ctypes.POINTER(ctypes.c_float)]
     lib.fm_options_default.restype = None
     lib.fm_options_default.argtypes = [ctypes.POINTER(FmOptions)]
     lib.fm_plan_create.restype = c_int

@@ -132,6 +132,24 @@ fm_status fm_plan_run(fm_plan plan, const void* A, const void* B, void* C, void*
 fm_status fm_plan_workspace(fm_plan plan, size_t* bytes);
 void fm_plan_destroy(fm_plan plan);
 
+/* fm_fastmatmul for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * reference counterpart: the reference allocates its temporaries on every
+ * call (algorithms/strassen.hpp:414-420).  It calls no hipMalloc, hipFree or
+ * allocating library routine: the plan's descriptors and workspace live in a
+ * fixed device array of FM_NOALLOC_SCRATCH_BYTES that the library holds on
+ * every device it runs on until the process ends, and a multiply needing more
+ * fails with FM_ERR_WORKSPACE (use fm_fastmatmul with a workspace then, or
+ * fewer levels).  Options are the process defaults.  All calls on one device
+ * share the array and leave their contents in it: calls that overlap on one
+ * device (two streams, or threads without ordering) race and must not be
+ * issued. */
+#define FM_NOALLOC_SCRATCH_BYTES 268435456u /* 256 MiB */
+fm_status fm_fastmatmul_noalloc(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k,
+                                int n, const void* A, int64_t lda, const void* B, int64_t ldb,
+                                void* C, int64_t ldc, int num_steps, double x, double alpha,
+                                double beta, void* stream, float* ms_out);
+
 /* Classical C := alpha * A * B + beta * C on the same leaf kernels
  * (MatMul, linalg/linalg.hpp:261-269), with the same leaf modes. */
 fm_status fm_gemm(fm_dtype dtype, fm_mode mode, int m, int k, int n, const void* A,
