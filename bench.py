@@ -14,6 +14,12 @@ events on the launch stream over the timed region) and the reference's CPU
 path timed on this host's cores.
 
     python bench.py [--steps K] [--warmup W] [--workload c2sq|c2ts|c1] [--no-cpu]
+                    [--configs c1,c2ts,c3,c4,c5 | --no-configs]
+
+Each secondary configuration reports its own roofline: the leaf kernel
+against its matrix-unit peak (FP64, FP32, BF16 x 6 slice products, INT8 x 36
+digit products) and every stage, including the plane preparation, against
+HBM bandwidth, from the library's per-launch events.
 """
 from __future__ import annotations
 
@@ -55,6 +61,7 @@ WORKLOADS = {
     # name: (alg, steps, M, K, N)
     "c2sq": ("grey424_26_206", 2, 8192, 8192, 8192),
     "c2ts": ("grey424_26_206", 2, 131072, 2400, 2400),
+    "c2ts1": ("grey424_26_206", 1, 131072, 2400, 2400),
     "c1": ("strassen", 1, 2048, 2048, 2048),
 }
 
@@ -89,6 +96,82 @@ def _err(C, D, A, B):
     return nw, float((d / D.abs()).max())
 
 
+# published dense peaks of MI355X: TFLOP/s (TOP/s for int8), HBM GB/s
+PEAKS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "int8": 5000.0, "hbm_GBps": PEAK_HBM_GBS}
+# slice products per leaf product: split-bf16 (3 slices, i + j <= 2) and
+# int8 digits (8 digits, s + t <= 7)
+SPLIT_PRODUCTS = 6
+OZAKI_PRODUCTS = 36
+
+
+def profiled(fastmm, fn, steps, sync):
+    """Per-step device time and launch count of each stage kind over `steps`
+    calls of fn (events the library records around each launch)."""
+    fn()
+    sync()
+    fastmm.profile_read()
+    fastmm.profile_enable(True)
+    try:
+        for _ in range(steps):
+            fn()
+        sync()
+    finally:
+        fastmm.profile_enable(False)
+    p = fastmm.profile_read()
+    return {k: (v[0] / steps, v[1] / steps) for k, v in p.items()}
+
+
+def roofline_block(prof, plan, leaf_ops, leaf_peak, leaf_unit, leaf_kind):
+    """Leaf kernel against its matrix-unit peak, each stage against HBM:
+    algorithmic work per step over the measured per-step time."""
+    out = {}
+    ms, n = prof["LEAF"]
+    if ms > 0:
+        a = leaf_ops / (ms * 1e-3) / 1e12
+        out["leaf"] = {"kernel": leaf_kind, "ms": ms, "launches": n, "achieved": a, "peak": leaf_peak,
+                       "unit": leaf_unit, "frac": a / leaf_peak, "ops": leaf_ops}
+    for k in ("S", "T", "W"):
+        ms, n = prof[k]
+        if ms > 0 and plan["stage_bytes"][k] > 0:
+            gbs = plan["stage_bytes"][k] / (ms * 1e-3) / 1e9
+            out[k] = {"ms": ms, "launches": n, "algorithmic_GB": plan["stage_bytes"][k] / 1e9,
+                      "GB_per_s": gbs, "hbm_frac": gbs / PEAK_HBM_GBS}
+    ms, n = prof.get("PREP", (0.0, 0))
+    if ms > 0:
+        blk = {"ms": ms, "launches": n}
+        if plan.get("prep_bytes"):
+            gbs = plan["prep_bytes"] / (ms * 1e-3) / 1e9
+            blk.update({"algorithmic_GB": plan["prep_bytes"] / 1e9, "GB_per_s": gbs,
+                        "hbm_frac": gbs / PEAK_HBM_GBS})
+        out["PREP"] = blk
+    if prof["PEEL"][0] > 0:
+        out["PEEL"] = {"ms": prof["PEEL"][0], "launches": prof["PEEL"][1]}
+    return out
+
+
+def measured_peaks(fastmm, torch, dev):
+    """The matrix units' sustained rates (fm_probe_mfma_peak) and the HBM
+    bandwidth of plain streaming kernels on this GPU."""
+    out = {}
+    for kind in ("fp64", "fp32", "bf16", "int8"):
+        try:
+            out[kind] = fastmm.probe_mfma_peak(kind, 0.05)
+        except Exception as e:  # reported, never fatal
+            out[kind] = "error: %s" % e
+    x = torch.empty(1 << 29, dtype=torch.float64, device=dev).fill_(1.0)
+    y = torch.empty_like(x)
+    sync = torch.cuda.synchronize
+    # one read and one write stream, as the stage kernels have
+    out["hbm_elementwise_GBps"] = 2.0 * x.numel() * 8 / timed(lambda: torch.abs(x, out=y), 5, sync) / 1e9
+    out["hbm_read_GBps"] = x.numel() * 8 / timed(lambda: x.sum(), 5, sync) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    out["note"] = ("tera-ops/s of back-to-back MFMAs on every SIMD (no memory traffic); GB/s of an "
+                   "elementwise 4 GiB read + 4 GiB write and of a 4 GiB read (torch kernels); "
+                   "the rooflines use the published peaks")
+    return out
+
+
 def other_configs(fastmm, torch, dev, names=("c1", "c2ts", "c3", "c4", "c5")):
     """Secondary measurements of the other configurations (those in
     `names`), each guarded."""
@@ -107,17 +190,36 @@ def other_configs(fastmm, torch, dev, names=("c1", "c2ts", "c3", "c4", "c5")):
             out[name] = {"error": "%s: %s" % (type(e).__name__, e)}
         torch.cuda.empty_cache()
 
+    def fast_roofline(alg, steps, M, K, N, fn, mode="plain", dtype=torch.float64, psteps=3):
+        t = fastmm.tables.get_table(alg)
+        F, _ = leaf_flops(t.R, t.m, t.k, t.n, steps, M, K, N)
+        plan = fastmm.plan_info(alg, M, K, N, steps, dtype=dtype, mode=mode)
+        prof = profiled(fastmm, fn, psteps, sync)
+        if mode == "split_bf16":
+            return roofline_block(prof, plan, F * SPLIT_PRODUCTS, PEAKS["bf16"], "TFLOP/s",
+                                  "split-bf16 slice products (6 per leaf, bf16 MFMA)")
+        if mode == "ozaki_int8":
+            return roofline_block(prof, plan, F * OZAKI_PRODUCTS, PEAKS["int8"], "TOP/s",
+                                  "int8 digit products (36 per leaf, int8 MFMA)")
+        key = "fp64" if dtype == torch.float64 else "fp32"
+        return roofline_block(prof, plan, F, PEAKS[key], "TFLOP/s", "%s MFMA leaf GEMM" % key.upper())
+
     def c1():
         n = 2048
         A, B = rnd(n, n, torch.float64), rnd(n, n, torch.float64)
         C = fastmm.empty(n, n, torch.float64, dev)
         fl = 2.0 * n ** 3
-        v, ms = _rate(lambda: fastmm.fastmatmul("strassen", A, B, C, 1), fl, sync, 10)
+        fn = lambda: fastmm.fastmatmul("strassen", A, B, C, 1)  # noqa: E731
+        v, ms = _rate(fn, fl, sync, 10)
         D = torch.matmul(B.t(), A.t()).t()
         vb, _ = _rate(lambda: torch.matmul(B.t(), A.t()), fl, sync, 10)
+        vo, _ = _rate(lambda: fastmm.gemm(A, B, C), fl, sync, 10)
+        fn()
         nw, ew = _err(C, D, A, B)
         return {"workload": "strassen numsteps=1 FP64 2048^3", "tflops": v, "ms": ms,
-                "torch_matmul_tflops": vb, "normwise_err": nw, "max_rel_diff": ew}
+                "torch_matmul_tflops": vb, "fastmm_gemm_tflops": vo, "normwise_err": nw,
+                "max_rel_diff": ew,
+                "roofline": fast_roofline("strassen", 1, n, n, n, fn, psteps=10)}
 
     def c2ts():
         M, K, N = 131072, 2400, 2400
@@ -125,14 +227,14 @@ def other_configs(fastmm, torch, dev, names=("c1", "c2ts", "c3", "c4", "c5")):
         C = fastmm.empty(M, N, torch.float64, dev)
         fl = 2.0 * M * K * N
         r = {"workload": "grey424_26_206 FP64 131072x2400x2400"}
+        D = torch.matmul(B.t(), A.t()).t()
         for st in (1, 2):
-            r["numsteps%d_tflops" % st], _ = _rate(
-                lambda: fastmm.fastmatmul("grey424_26_206", A, B, C, st), fl, sync)
+            fn = lambda: fastmm.fastmatmul("grey424_26_206", A, B, C, st)  # noqa: E731
+            r["numsteps%d_tflops" % st], r["numsteps%d_ms" % st] = _rate(fn, fl, sync)
+            r["numsteps%d_normwise_err" % st], r["numsteps%d_max_rel_diff" % st] = _err(C, D, A, B)
+            r["numsteps%d_roofline" % st] = fast_roofline("grey424_26_206", st, M, K, N, fn)
         r["fastmm_gemm_tflops"], _ = _rate(lambda: fastmm.gemm(A, B, C), fl, sync)
         r["torch_matmul_tflops"], _ = _rate(lambda: torch.matmul(B.t(), A.t()), fl, sync)
-        D = torch.matmul(B.t(), A.t()).t()
-        fastmm.fastmatmul("grey424_26_206", A, B, C, 2)
-        r["normwise_err_numsteps2"], r["max_rel_diff_numsteps2"] = _err(C, D, A, B)
         return r
 
     def c4():
@@ -147,17 +249,23 @@ def other_configs(fastmm, torch, dev, names=("c1", "c2ts", "c3", "c4", "c5")):
                 B = fastmm.col_major((B * cs).contiguous())
             C = fastmm.empty(n, n, torch.float32, dev)
             fl = 2.0 * n ** 3
-            v, ms = _rate(lambda: fastmm.fastmatmul("strassen", A, B, C, 2, mode="split_bf16"), fl, sync)
+            fn = lambda: fastmm.fastmatmul("strassen", A, B, C, 2, mode="split_bf16")  # noqa: E731
+            v, ms = _rate(fn, fl, sync)
             # FP64 product as the oracle, on a 2048-row slice
             rows = slice(0, 2048)
             D = torch.matmul(B.double().t(), A[rows].double().t()).t()
             nw, ew = _err(C[rows], D, A[rows].double(), B.double())
             r[dist] = {"tflops": v, "ms": ms, "normwise_err_vs_fp64": nw, "max_rel_diff_vs_fp64": ew}
+            if dist == "uniform":
+                r["roofline"] = fast_roofline("strassen", 2, n, n, n, fn, mode="split_bf16",
+                                              dtype=torch.float32)
             fastmm.fastmatmul("strassen", A, B, C, 2)
             nwp, _ = _err(C[rows], D, A[rows].double(), B.double())
             r[dist]["plain_fp32_leaves_normwise_err"] = nwp
             del D
-        r["plain_fp32_strassen2_tflops"], _ = _rate(lambda: fastmm.fastmatmul("strassen", A, B, C, 2), 2.0 * n ** 3, sync)
+        fn = lambda: fastmm.fastmatmul("strassen", A, B, C, 2)  # noqa: E731
+        r["plain_fp32_strassen2_tflops"], _ = _rate(fn, 2.0 * n ** 3, sync)
+        r["plain_fp32_roofline"] = fast_roofline("strassen", 2, n, n, n, fn, dtype=torch.float32)
         r["torch_fp32_matmul_tflops"], _ = _rate(lambda: torch.matmul(B.t(), A.t()), 2.0 * n ** 3, sync)
         return r
 
@@ -167,12 +275,15 @@ def other_configs(fastmm, torch, dev, names=("c1", "c2ts", "c3", "c4", "c5")):
         C = fastmm.empty(n, n, torch.float64, dev)
         fl = 2.0 * n ** 3
         r = {"workload": "smirnov333_23_139 numsteps=2 FP64 13824^3, int8-digit leaves (8 digits)"}
-        r["ozaki_tflops"], _ = _rate(
-            lambda: fastmm.fastmatmul("smirnov333_23_139", A, B, C, 2, mode="ozaki_int8"), fl, sync)
+        fn = lambda: fastmm.fastmatmul("smirnov333_23_139", A, B, C, 2, mode="ozaki_int8")  # noqa: E731
+        r["ozaki_tflops"], r["ozaki_ms"] = _rate(fn, fl, sync)
         D = torch.matmul(B.t(), A.t()).t()
         r["ozaki_normwise_err"], r["ozaki_max_rel_diff"] = _err(C, D, A, B)
-        r["plain_fp64_tflops"], _ = _rate(lambda: fastmm.fastmatmul("smirnov333_23_139", A, B, C, 2), fl, sync)
+        r["ozaki_roofline"] = fast_roofline("smirnov333_23_139", 2, n, n, n, fn, mode="ozaki_int8")
+        fn = lambda: fastmm.fastmatmul("smirnov333_23_139", A, B, C, 2)  # noqa: E731
+        r["plain_fp64_tflops"], r["plain_fp64_ms"] = _rate(fn, fl, sync)
         r["plain_normwise_err"], _ = _err(C, D, A, B)
+        r["plain_fp64_roofline"] = fast_roofline("smirnov333_23_139", 2, n, n, n, fn)
         r["torch_matmul_tflops"], _ = _rate(lambda: torch.matmul(B.t(), A.t()), fl, sync)
         return r
 
@@ -187,31 +298,50 @@ def other_configs(fastmm, torch, dev, names=("c1", "c2ts", "c3", "c4", "c5")):
             probs.append({"A": rnd(m, k, torch.float32), "B": rnd(k, n, torch.float32),
                           "C": fastmm.empty(m, n, torch.float32, dev)})
             fl += 2.0 * m * k * n
+
         def batch_run(mode):
             b = fastmm.Batch(probs, mode=mode, graph=True)
             v, ms = _rate(b.run, fl, sync)
             chosen = {}
-            for i in range(len(probs)):
+            leaf_fl = 0.0
+            for i, (m, k, n) in enumerate(shapes):
                 a, st = b.choice(i)
                 chosen["%s/%d" % (a, st)] = chosen.get("%s/%d" % (a, st), 0) + 1
+                if st == 0:
+                    leaf_fl += 2.0 * m * k * n
+                else:
+                    t = fastmm.tables.get_table(a)
+                    leaf_fl += leaf_flops(t.R, t.m, t.k, t.n, st, m, k, n)[0]
             errs = []
             for p in probs[:8]:
                 D = torch.matmul(p["B"].double().t(), p["A"].double().t()).t()
                 errs.append(_err(p["C"], D, p["A"].double(), p["B"].double())[0])
             del b
-            return v, ms, chosen, max(errs)
+            # the same batch without the graph, with the library's per-launch events
+            be = fastmm.Batch(probs, mode=mode, graph=False)
+            prof = profiled(fastmm, be.run, 3, sync)
+            del be
+            plan = {"stage_bytes": {"S": 0.0, "T": 0.0, "W": 0.0}}
+            if mode == "split_bf16":
+                roof = roofline_block(prof, plan, leaf_fl * SPLIT_PRODUCTS, PEAKS["bf16"], "TFLOP/s",
+                                      "split-bf16 slice products (6 per leaf, bf16 MFMA)")
+            else:
+                roof = roofline_block(prof, plan, leaf_fl, PEAKS["fp32"], "TFLOP/s",
+                                      "FP32 MFMA grouped leaf GEMM")
+            return v, ms, chosen, max(errs), roof
 
         # FP32-accurate leaves from split-bf16 slice products (6 bf16 MFMA
         # products per leaf, FP32 accumulation)
-        vs, mss, chosen_s, err_s = batch_run("split_bf16")
-        v, ms, chosen, err_p = batch_run("plain")
+        vs, mss, chosen_s, err_s, roof_s = batch_run("split_bf16")
+        v, ms, chosen, err_p, roof_p = batch_run("plain")
         # classical reference: one torch matmul per problem
         vt, _ = _rate(lambda: [torch.matmul(p["B"].t(), p["A"].t()) for p in probs], fl, sync)
         return {"workload": "256 FP32 problems, M,K,N uniform on {512,...,4096}/64, autotuned, hipGraph",
                 "total_gflop": fl / 1e9, "tflops": v, "ms": ms, "choices": chosen,
                 "torch_per_problem_matmul_tflops": vt, "normwise_err_max_first8": err_p,
+                "roofline": roof_p,
                 "split_bf16": {"tflops": vs, "ms": mss, "choices": chosen_s,
-                               "normwise_err_max_first8": err_s}}
+                               "normwise_err_max_first8": err_s, "roofline": roof_s}}
 
     for name, fn in (("c1", c1), ("c2ts", c2ts), ("c3", c3), ("c4", c4), ("c5", c5)):
         if name in names:
@@ -226,7 +356,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2sq", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the host CPU baseline")
-    ap.add_argument("--cpu-n", type=int, default=4096, help="square size of the CPU sample")
+    ap.add_argument("--cpu-n", type=int, default=8192,
+                    help="square size of the all-core CPU modes (the GPU workload's size)")
+    ap.add_argument("--cpu-n1", type=int, default=4096, help="square size of the one-core CPU mode")
+    ap.add_argument("--cpu-reps", type=int, default=5, help="timed CPU runs per mode (median)")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the secondary measurements of the other workloads")
     ap.add_argument("--configs", default="c1,c2ts,c3,c4,c5",
@@ -311,6 +444,8 @@ def main():
         sync()
         del x, y
 
+    peaks = measured_peaks(fastmm, torch, dev)
+
     configs = None
     if not args.no_configs and args.workload == "c2sq":
         del Cg, D, At, Bt
@@ -321,8 +456,8 @@ def main():
     if not args.no_cpu:
         try:
             r = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"),
-                                "--alg", alg, "--steps", str(steps), "--m", str(args.cpu_n),
-                                "--k", str(args.cpu_n), "--n", str(args.cpu_n), "--reps", "1"],
+                                "--alg", alg, "--steps", str(steps), "--n", str(args.cpu_n),
+                                "--n1", str(args.cpu_n1), "--reps", str(args.cpu_reps)],
                                stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True,
                                timeout=600)
             if r.returncode == 0:
@@ -365,6 +500,7 @@ def main():
                      "traffic_source": os.path.relpath(TRAFFIC_FILE, ROOT)
                      if args.workload == "c2sq" else None},
         "cpu_baseline": cpu,
+        "measured_peaks": peaks,
         "configs": configs,
     }
     # whole multiply against its roofline: leaf flops at the FP64 MFMA peak

@@ -11,6 +11,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Phases of a leaf launch on prepared planes (split-bf16 slices, int8
+// digits): the preparation kernels that cut the operands into planes, the
+// products on them, or both.
+enum FmLeafPhase : int { FM_PHASE_PREP = 1, FM_PHASE_PRODUCT = 2, FM_PHASE_ALL = 3 };
+
 enum FmRole : int32_t { FM_ROLE_A = 0, FM_ROLE_B = 1, FM_ROLE_C = 2, FM_ROLE_WS = 3 };
 
 // Roles 0-3 come from p[]; a batched plan numbers its problems' operands

@@ -615,6 +615,14 @@ fm_status fm_plan_query(const char* alg_name, fm_dtype dtype, fm_mode mode, int 
     info->launches[L.stage] += 1;
     info->stage_bytes[L.stage] += launch_bytes(L, es);
     if (L.kind == LK_S2 || L.kind == LK_T2 || L.kind == LK_W2) info->two_level_stages += 1;
+    if (L.kind == LK_SPLIT)
+      for (const FmSplitPrep& r : L.spreps)
+        info->prep_bytes += es * (double)r.rows * r.K + 6.0 * r.rows_pad * r.Kp;
+    if (L.kind == LK_OZAKI)
+      for (const FmOzakiDesc& d : L.odescs)
+        info->prep_bytes += es * ((double)d.M * d.K + (double)d.K * d.N) +
+                            (double)L.oz_slices * ((double)d.rows_pad + d.cols_pad) * d.Kp +
+                            4.0 * (d.rows_pad + d.cols_pad);
     if (L.stage == FM_STAGE_LEAF) {
       const int tiles = L.kind == LK_OZAKI   ? L.oz.total_tiles
                         : L.kind == LK_SPLIT ? L.sp.total_tiles

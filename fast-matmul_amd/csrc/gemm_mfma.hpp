@@ -143,5 +143,5 @@ struct FmSplitArgs {
   int32_t even_k;               // 1: every product has an even number of K slices
 };
 
-// the preparation launch, then the grouped product launch
-hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, hipStream_t s);
+// the preparation launch, then the grouped product launch (phases: FmLeafPhase)
+hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, int phases, hipStream_t s);

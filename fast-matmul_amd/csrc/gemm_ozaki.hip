@@ -426,11 +426,15 @@ __global__ void __launch_bounds__(256, 2) fm_ozaki_gemm32(FmOzakiArgs args) {
 }
 
 template <int S>
-static hipError_t fm_ozaki_run(const FmOzakiArgs& a, int max_rows, int max_cols, hipStream_t s) {
-  hipLaunchKernelGGL((fm_ozaki_prep_rows<S>), dim3((max_rows + 63) / 64, a.ndesc), dim3(256), 0,
-                     s, a);
-  hipLaunchKernelGGL((fm_ozaki_prep_cols<S>), dim3((max_cols + 63) / 64, a.ndesc), dim3(256), 0,
-                     s, a);
+static hipError_t fm_ozaki_run(const FmOzakiArgs& a, int max_rows, int max_cols, int phases,
+                               hipStream_t s) {
+  if (phases & FM_PHASE_PREP) {
+    hipLaunchKernelGGL((fm_ozaki_prep_rows<S>), dim3((max_rows + 63) / 64, a.ndesc), dim3(256), 0,
+                       s, a);
+    hipLaunchKernelGGL((fm_ozaki_prep_cols<S>), dim3((max_cols + 63) / 64, a.ndesc), dim3(256), 0,
+                       s, a);
+  }
+  if (!(phases & FM_PHASE_PRODUCT)) return hipGetLastError();
   if (a.variant == 0)
     hipLaunchKernelGGL((fm_ozaki_gemm<S>), dim3(a.total_tiles), dim3(256), 0, s, a);
   else
@@ -439,12 +443,12 @@ static hipError_t fm_ozaki_run(const FmOzakiArgs& a, int max_rows, int max_cols,
 }
 
 hipError_t fm_ozaki_launch(int slices, const FmOzakiArgs& a, int max_rows, int max_cols,
-                           hipStream_t s) {
+                           int phases, hipStream_t s) {
   if (a.total_tiles <= 0) return hipSuccess;
   switch (slices) {
-    case 6: return fm_ozaki_run<6>(a, max_rows, max_cols, s);
-    case 7: return fm_ozaki_run<7>(a, max_rows, max_cols, s);
-    case 8: return fm_ozaki_run<8>(a, max_rows, max_cols, s);
+    case 6: return fm_ozaki_run<6>(a, max_rows, max_cols, phases, s);
+    case 7: return fm_ozaki_run<7>(a, max_rows, max_cols, phases, s);
+    case 8: return fm_ozaki_run<8>(a, max_rows, max_cols, phases, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -31,4 +31,4 @@ struct FmOzakiArgs {
 inline int64_t fm_ozaki_max_kp(int slices) { return (int64_t)2147483647 / (slices * 127 * 127); }
 
 hipError_t fm_ozaki_launch(int slices, const FmOzakiArgs& a, int max_rows, int max_cols,
-                           hipStream_t s);
+                           int phases, hipStream_t s);

@@ -633,8 +633,8 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm_bd(FmSplitArgs arg
   }
 }
 
-hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, hipStream_t s) {
-  if (a.nprep > 0 && a.prep_bx > 0 && a.prep_by > 0) {
+hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, int phases, hipStream_t s) {
+  if ((phases & FM_PHASE_PREP) && a.nprep > 0 && a.prep_bx > 0 && a.prep_by > 0) {
     for (int first = 0; first < a.nprep; first += 65535) {
       const int nz = a.nprep - first < 65535 ? a.nprep - first : 65535;
       hipLaunchKernelGGL(fm_split_prep,
@@ -644,7 +644,7 @@ hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, hipStream_t s) {
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  if (a.total_tiles <= 0) return hipSuccess;
+  if (!(phases & FM_PHASE_PRODUCT) || a.total_tiles <= 0) return hipSuccess;
   const int grid = a.grid > 0 && a.grid < a.total_tiles ? a.grid : a.total_tiles;
   switch (cfg) {
     case FM_TILE_SP_128x256_BD:  // one work group per tile

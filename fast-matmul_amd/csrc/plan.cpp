@@ -971,14 +971,30 @@ fm_status Plan::run(const void* A, const void* B, void* C, void* ws, hipStream_t
     }
     if (L.kind == LK_SPLIT && ((uintptr_t)ws % 16 != 0))
       return set_err(FM_ERR_INVALID_ARG, "split_bf16 leaves need a 16-byte aligned workspace");
-    if (L.kind == LK_OZAKI) {
-      FmOzakiArgs o = L.oz;
-      o.bases = bases;
-      e = fm_ozaki_launch(L.oz_slices, o, L.oz_rows, L.oz_cols, s);
-    } else if (L.kind == LK_SPLIT) {
-      FmSplitArgs a = L.sp;
-      a.bases = bases;
-      e = fm_split_launch(L.cfg, a, s);
+    if (L.kind == LK_OZAKI || L.kind == LK_SPLIT) {
+      // profiled: the plane preparation and the products are timed apart
+      // (preparation as FM_STAGE_PREP)
+      auto go = [&](int phases) {
+        if (L.kind == LK_OZAKI) {
+          FmOzakiArgs o = L.oz;
+          o.bases = bases;
+          return fm_ozaki_launch(L.oz_slices, o, L.oz_rows, L.oz_cols, phases, s);
+        }
+        FmSplitArgs a = L.sp;
+        a.bases = bases;
+        return fm_split_launch(L.cfg, a, phases, s);
+      };
+      if (g_prof_on && pr.e0 && pr.e1) {
+        e = go(FM_PHASE_PREP);
+        (void)hipEventRecord(pr.e1, s);
+        g_prof_pending.push_back(ProfRec{FM_STAGE_PREP, pr.e0, pr.e1});
+        pr.e0 = pool_event();
+        pr.e1 = pool_event();
+        if (pr.e0) (void)hipEventRecord(pr.e0, s);
+        if (e == hipSuccess) e = go(FM_PHASE_PRODUCT);
+      } else {
+        e = go(FM_PHASE_ALL);
+      }
     } else if (L.kind == LK_GEMM) {
       FmGemmArgs g = L.gm;
       g.bases = bases;

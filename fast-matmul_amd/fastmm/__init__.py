@@ -282,12 +282,14 @@ def plan_info(alg_name: str, m: int, k: int, n: int, num_steps: int = 1,
     info = _lib.FmPlanInfo()
     check(LIB.fm_plan_query(alg_name.encode(), _DT[dtype], _lib.MODES[mode], int(m), int(k),
                             int(n), int(num_steps), ctypes.byref(info)))
-    return {"launches": {nm: info.launches[i] for i, nm in enumerate(_lib.STAGES)},
-            "stage_bytes": {nm: info.stage_bytes[i] for i, nm in enumerate(_lib.STAGES)},
+    kinds = _lib.STAGES[:5]  # plan stages; plane preparation counts with LEAF
+    return {"launches": {nm: info.launches[i] for i, nm in enumerate(kinds)},
+            "stage_bytes": {nm: info.stage_bytes[i] for i, nm in enumerate(kinds)},
             "two_level_stages": info.two_level_stages,
             "leaf": [{"cfg": info.leaf_cfg[i], "products": info.leaf_products[i],
                       "tiles": info.leaf_tiles[i]} for i in range(min(2, info.launches[3]))],
-            "workspace_bytes": info.workspace_bytes, "dfs_levels": info.dfs_levels}
+            "workspace_bytes": info.workspace_bytes, "dfs_levels": info.dfs_levels,
+            "prep_bytes": info.prep_bytes}
 
 
 def set_schedule(dfs_levels: int = 0, workspace_limit: int = 0, split_remainder: bool = True):
@@ -312,6 +314,22 @@ def set_ozaki_slices(slices: int = 8):
 def profile_enable(on: bool = True):
     """Bracket every library launch with HIP events on its stream."""
     LIB.fm_profile_enable(1 if on else 0)
+
+
+_PEAK_KINDS = {"fp64": 0, "fp32": 1, "bf16": 2, "int8": 3}
+# published dense peaks of MI355X (TFLOP/s, TOP/s for int8), for sizing the probe
+_PUBLISHED = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0, "int8": 5000.0}
+
+
+def probe_mfma_peak(kind: str, seconds: float = 0.05) -> float:
+    """Measured matrix-unit peak (tera-ops/s) of one MFMA kind on this GPU,
+    running about ``seconds`` (fm_probe_mfma_peak)."""
+    # operations per probe iteration: 256 CUs x 8 work groups x 4 waves x 8 chains
+    per_iter = {"fp64": 2048, "fp32": 2048, "bf16": 16384, "int8": 65536}[kind] * 8 * 4 * 8 * 256
+    iters = max(1, int(_PUBLISHED[kind] * 1e12 * seconds / per_iter))
+    out = ctypes.c_double(0.0)
+    check(LIB.fm_probe_mfma_peak(_PEAK_KINDS[kind], iters, ctypes.byref(out)))
+    return out.value
 
 
 def profile_read():

@@ -34,7 +34,7 @@ EXPORTS = (
     "fm_set_stage_fusion", "fm_set_schedule", "fm_plan_query", "fm_fastmatmul_scaled",
     "fm_options_default", "fm_fastmatmul_ex", "fm_plan_create", "fm_plan_run",
     "fm_plan_workspace", "fm_plan_destroy", "fm_scaled_workspace_query", "fm_release_workspace",
-    "fm_fastmatmul_noalloc",
+    "fm_fastmatmul_noalloc", "fm_probe_mfma_peak",
 )
 
 # the fixed scratch array of fm_fastmatmul_noalloc (FM_NOALLOC_SCRATCH_BYTES)
@@ -43,7 +43,7 @@ NOALLOC_SCRATCH_BYTES = 256 << 20
 # scaling rules of scaling/scaling.hpp:17-22
 SCALINGS = {"outside_inside": 0, "inside_outside": 1, "inside": 2, "outside": 3}
 
-STAGES = ("S", "T", "W", "LEAF", "PEEL")
+STAGES = ("S", "T", "W", "LEAF", "PEEL", "PREP")
 
 
 class FmPlanInfo(ctypes.Structure):
@@ -51,7 +51,7 @@ class FmPlanInfo(ctypes.Structure):
                 ("two_level_stages", ctypes.c_int),
                 ("leaf_cfg", ctypes.c_int * 2), ("leaf_products", ctypes.c_int * 2),
                 ("leaf_tiles", ctypes.c_int * 2), ("workspace_bytes", ctypes.c_size_t),
-                ("dfs_levels", ctypes.c_int)]
+                ("dfs_levels", ctypes.c_int), ("prep_bytes", ctypes.c_double)]
 
 
 class FmOptions(ctypes.Structure):
@@ -61,7 +61,7 @@ class FmOptions(ctypes.Structure):
 
 
 class FmProfile(ctypes.Structure):
-    _fields_ = [("ms", ctypes.c_double * 5), ("launches", ctypes.c_int64 * 5)]
+    _fields_ = [("ms", ctypes.c_double * 6), ("launches", ctypes.c_int64 * 6)]
 
 
 class FmProblem(ctypes.Structure):
@@ -99,6 +99,8 @@ def _load():
     lib.fm_fastmatmul_noalloc.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int,
                                           vp, i64, vp, i64, vp, i64, c_int, c_double, c_double,
                                           c_double, vp, ctypes.POINTER(ctypes.c_float)]
+    lib.fm_probe_mfma_peak.restype = c_int
+    lib.fm_probe_mfma_peak.argtypes = [c_int, c_int, ctypes.POINTER(c_double)]
     lib.fm_options_default.restype = None
     lib.fm_options_default.argtypes = [ctypes.POINTER(FmOptions)]
     lib.fm_plan_create.restype = c_int

@@ -218,7 +218,8 @@ typedef enum {
   FM_STAGE_W = 2,     /* C_ij = sum W M (+ beta C)  (HBM-bound)             */
   FM_STAGE_LEAF = 3,  /* grouped leaf products (MFMA-bound)                 */
   FM_STAGE_PEEL = 4,  /* dynamic-peeling products                           */
-  FM_STAGE_COUNT = 5
+  FM_STAGE_PREP = 5,  /* split-bf16 / int8-digit plane preparation (HBM)    */
+  FM_STAGE_COUNT = 6
 } fm_stage_kind;
 
 typedef struct {
@@ -228,6 +229,14 @@ typedef struct {
 
 void fm_profile_enable(int on);
 fm_status fm_profile_read(fm_profile* out);
+
+/* Measured matrix-unit peak of the current device, for roofline figures (no
+ * reference counterpart): every SIMD issues `iters` rounds of independent
+ * back-to-back MFMAs on register operands; the result is in tera-operations
+ * per second (2 per multiply-add) at the clock the chip holds under that
+ * load.  kind 0: FP64 (v_mfma_f64_16x16x4f64), 1: FP32 (16x16x4f32),
+ * 2: BF16 (16x16x32_bf16, FP32 accumulate), 3: INT8 (32x32x32_i8). */
+fm_status fm_probe_mfma_peak(int kind, int iters, double* tera_ops_per_s);
 
 /* Process default of fm_options.ozaki_slices.
  * Digit count S of FM_MODE_OZAKI_INT8 leaves (6, 7 or 8; default 8): each
@@ -272,14 +281,15 @@ fm_status fm_scaled_workspace_query(const char* alg, fm_dtype dtype, fm_mode mod
  * first two leaf launches' tile configuration (-1: int8-digit leaves),
  * product and tile counts. */
 typedef struct fm_plan_info {
-  int launches[5];
+  int launches[5];       /* FM_STAGE_S .. FM_STAGE_PEEL (preparation counts with LEAF) */
   double stage_bytes[5];
   int two_level_stages;
   int leaf_cfg[2];
   int leaf_products[2];
   int leaf_tiles[2];
   size_t workspace_bytes;
-  int dfs_levels; /* depth-first top levels the plan runs */
+  int dfs_levels;    /* depth-first top levels the plan runs */
+  double prep_bytes; /* split-bf16 / int8-digit plane preparation: operands read, planes written */
 } fm_plan_info;
 fm_status fm_plan_query(const char* alg, fm_dtype dtype, fm_mode mode, int m, int k, int n,
                         int num_steps, fm_plan_info* info);

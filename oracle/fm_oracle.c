@@ -43,6 +43,17 @@
 #include <string.h>
 #include <quadmath.h>
 
+/* Built with -fopenmp -DFMO_OMP_ADDS (liboracle_par.so, for the CPU
+ * baseline only), the write-once adds split their columns over OpenMP
+ * threads: the reference's _PARALLEL_ DFS mode runs its add loops as OpenMP
+ * parallel-for loops (algorithms/strassen.hpp:17-19).  Each element's sum
+ * is unchanged, so results are identical either way. */
+#ifdef FMO_OMP_ADDS
+#define FMO_PAR_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define FMO_PAR_FOR
+#endif
+
 typedef void (*dgemm_fn)(const char*, const char*, const int*, const int*, const int*,
                          const double*, const double*, const int*, const double*, const int*,
                          const double*, double*, const int*);
@@ -114,6 +125,7 @@ typedef struct {
   /* dst = sum_t coef_t * src_t in the reference's write-once form. */                      \
   static void add_##SUF(int nterm, mat_##SUF* src, const double* coef, mat_##SUF dst,        \
                         int with_beta, T beta) {                                             \
+    FMO_PAR_FOR                                                                              \
     for (int j = 0; j < dst.n; ++j) {                                                        \
       for (int i = 0; i < dst.m; ++i) {                                                      \
         T acc = 0;                                                                           \

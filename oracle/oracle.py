@@ -41,10 +41,23 @@ class FmoTable(ctypes.Structure):
                 ("SX", ctypes.c_void_p), ("TX", ctypes.c_void_p), ("MX", ctypes.c_void_p)]
 
 
+_lib_name = "liboracle.so"
+
+
+def use_parallel_adds():
+    """Load the build whose add loops run over OpenMP threads
+    (liboracle_par.so) instead of the sequential one; for the CPU baseline,
+    in its own process, before anything else loads the oracle."""
+    global _lib_name
+    if _lib is not None and _lib_name != "liboracle_par.so":
+        raise RuntimeError("the sequential oracle is already loaded in this process")
+    _lib_name = "liboracle_par.so"
+
+
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(HERE, "liboracle.so")
+        path = os.path.join(HERE, _lib_name)
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run `make -C oracle`")
         _lib = ctypes.CDLL(path)
