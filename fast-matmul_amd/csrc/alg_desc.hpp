@@ -53,6 +53,22 @@ struct FmOp {
 // Two-level W stage: ops[q] are the level-l C operands (blocks rows x cols,
 // sub-blocks rows2 x cols2); the level-(l+2) products of node q are at
 // ws + out_off + ((q * R + r) * R + r2) * out_slab.
+//
+// Grouped stage launch (the nodes of several problems, whose blocks differ
+// in shape, in one launch): `node` holds each node's operand, shape, output
+// placement and scalars, and the launch-wide fields hold the largest shape
+// (which sizes the grid).  A node's outputs start at ws + out_off with the
+// same slab layout as above for its own index 0.
+struct FmStageNode {
+  FmOp op;
+  int32_t rows, cols;    // blocks (one-level) / blocks of the level-l operand (two-level)
+  int32_t rows2, cols2;  // two-level stages: sub-blocks
+  int64_t out_off;       // the node's first output slab (workspace elements)
+  int64_t out_ld, out_slab;
+  int64_t view_off;      // two-level S/T: the node's first view slab
+  double x, beta;
+};
+
 struct FmStageArgs {
   FmBases bases;
   const FmOp* ops;
@@ -62,7 +78,50 @@ struct FmStageArgs {
   double beta;
   int32_t rows2, cols2;
   int64_t view_off;
+  const FmStageNode* node;  // grouped launch: per-node descriptors (nullptr otherwise)
 };
+
+// The placement of node q of a stage launch, uniform or grouped.  nout: output
+// slabs per node of the one-level layout (S/T: NS/NT; W: R; two-level S/T:
+// R * NOUT; two-level W: R * R); nview: view slabs per node (R * NV).
+struct FmNodeView {
+  FmOp op;
+  int32_t rows, cols, rows2, cols2;
+  int64_t out, out_ld, out_slab, view;
+  double x, beta;
+};
+
+__host__ __device__ __forceinline__ FmNodeView fm_node_view(const FmStageArgs& a, int q, int nout,
+                                                           int nview) {
+  FmNodeView v;
+  if (a.node) {
+    const FmStageNode& n = a.node[q];
+    v.op = n.op;
+    v.rows = n.rows;
+    v.cols = n.cols;
+    v.rows2 = n.rows2;
+    v.cols2 = n.cols2;
+    v.out = n.out_off;
+    v.out_ld = n.out_ld;
+    v.out_slab = n.out_slab;
+    v.view = n.view_off;
+    v.x = n.x;
+    v.beta = n.beta;
+  } else {
+    v.op = a.ops[q];
+    v.rows = a.rows;
+    v.cols = a.cols;
+    v.rows2 = a.rows2;
+    v.cols2 = a.cols2;
+    v.out = a.out_off + (int64_t)q * nout * a.out_slab;
+    v.out_ld = a.out_ld;
+    v.out_slab = a.out_slab;
+    v.view = a.view_off + (int64_t)q * nview * a.out_slab;
+    v.x = a.x;
+    v.beta = a.beta;
+  }
+  return v;
+}
 
 typedef hipError_t (*FmStageLaunch)(const FmStageArgs& a, hipStream_t s);
 

@@ -9,12 +9,14 @@
 // 4 + 3p (A), 5 + 3p (B), 6 + 3p (C) resolved through the plan's device
 // table, so one launch can address every problem's matrices.
 //
-// The autotuner prices every candidate (algorithm, depth) with a cost model
-// of this engine on MI355X -- leaf flops (tile-padded) over the measured
-// leaf rate, stage bytes over the measured stage bandwidth, plus a per-launch
-// cost -- and keeps the cheapest.  The reference's counterpart is its HYBRID
-// policy (util/par_util.hpp:17-95), which fixes the schedule of one
-// algorithm; here the algorithm and depth are chosen per shape as well.
+// The autotuner prices every candidate (algorithm, depth) with the cost
+// model of costmodel.hpp -- leaf flops (tile-padded) over the leaf rate
+// measured at that leaf size, stage bytes over the measured stage
+// bandwidth, a measured per-level and per-launch cost -- and keeps the
+// cheapest.  The rates come from a table that python -m fastmm.calibrate
+// writes on the GPU.  The reference's counterpart is its HYBRID policy
+// (util/par_util.hpp:17-95), which fixes the schedule of one algorithm;
+// here the algorithm and depth are chosen per shape as well.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "costmodel.hpp"
 #include "plan.hpp"
 
 using namespace fmplan;
@@ -34,64 +37,6 @@ extern const FmAlgDesc* fm_find_alg(const char* name);
 
 namespace {
 
-// Measured on MI355X (bench.py / sweeps): leaf GEMM rate in TFLOP/s of
-// 2mnk-equivalent work, stage bandwidth, launch cost, and the cost of one
-// recursion level of a batched problem beyond its stage bytes.  On the
-// batch of 256 FP32 problems of 512-4096 (plain leaves) every problem
-// given a level cost the batch 0.1-0.3 ms more than the bytes and launches
-// predicted: its small stage and peeling launches do not fill the GPU, and
-// its odd-shaped leaves take the grouped launch's edge paths.
-struct CostModel {
-  double leaf_tflops;
-  double stage_tbps;
-  double launch_us;
-  double level_us;
-};
-
-// Leaf rates (TFLOP/s of 2mnk) measured on MI355X: split-bf16 leaves 226 on
-// 49 products of 4096^3, int8-digit leaves 67 (FP64-equivalent) on 529 of
-// 1536^3; the model takes a little less for the smaller leaves it prices.
-CostModel model_for(fm_dtype dt, fm_mode mode) {
-  if (mode == FM_MODE_SPLIT_BF16) return {220.0, 5.0, 4.0, 250.0};
-  if (mode == FM_MODE_OZAKI_INT8) return {65.0, 5.0, 4.0, 250.0};
-  return dt == FM_F64 ? CostModel{60.0, 5.0, 4.0, 250.0} : CostModel{100.0, 5.0, 4.0, 250.0};
-}
-
-// Estimated device time (us) of one multiply with `alg` at `steps` levels.
-double estimate_us(fm_dtype dt, fm_mode mode, const FmAlgDesc* alg, int64_t m, int64_t k,
-                   int64_t n, int steps) {
-  const CostModel cm = model_for(dt, mode);
-  const double es = elem_size(dt);
-  auto padded = [](int64_t v) { return (double)align_up(v, 64); };
-  if (!alg || steps == 0) return 2.0 * padded(m) * padded(n) * k / (cm.leaf_tflops * 1e6) + cm.launch_us;
-  double us = 0.0;
-  int64_t rm = m, rk = k, rn = n;
-  double nodes = 1.0;
-  int launches = 1;  // leaves
-  for (int l = 0; l < steps; ++l) {
-    const int64_t bm = rm / alg->m, bk = rk / alg->k, bn = rn / alg->n;
-    if (bm <= 0 || bk <= 0 || bn <= 0) return INFINITY;
-    const double blkA = (double)bm * bk, blkB = (double)bk * bn, blkC = (double)bm * bn;
-    // S: read m*k blocks, write NS; T likewise; W: read R, write m*n
-    const double bytes = es * nodes *
-                         (blkA * (alg->m * alg->k + alg->ns) + blkB * (alg->k * alg->n + alg->nt) +
-                          blkC * (alg->R + alg->m * alg->n));
-    us += bytes / (cm.stage_tbps * 1e6);
-    // peeling products of this level's nodes
-    const double erA = (double)(rm % alg->m), ecA = (double)(rk % alg->k), ecB = (double)(rn % alg->n);
-    us += nodes * 2.0 * (ecA * (rm - erA) * (rn - ecB) + ecB * rm * rk + erA * rk * (rn - ecB)) /
-          (0.5 * cm.leaf_tflops * 1e6);
-    launches += 3 + (erA + ecA + ecB > 0 ? 1 : 0);
-    us += cm.level_us;
-    nodes *= alg->R;
-    rm = bm;
-    rk = bk;
-    rn = bn;
-  }
-  us += nodes * 2.0 * padded(rm) * padded(rn) * rk / (cm.leaf_tflops * 1e6);
-  return us + launches * cm.launch_us;
-}
-
 // Strassen, <4,2,3;20> in all six orientations (the reference ships each
 // permutation as its own table; for an uneven problem the one whose block
 // split matches the shape's aspect needs the fewest flops and no peeling)
@@ -100,6 +45,12 @@ const char* const kDefaultCandidates[] = {"strassen",       "grey423_20_144", "g
                                           "grey243_20_144", "grey234_20_144", "grey324_20_144",
                                           "grey342_20_144", "smirnov333_23_139"};
 constexpr int kNumDefaultCandidates = (int)(sizeof(kDefaultCandidates) / sizeof(kDefaultCandidates[0]));
+
+// Estimated time of a recursive candidate is multiplied by this before the
+// comparison (measured on 256 uneven FP32 problems: with no margin, the 19
+// problems the model gave to <4,2,3;20> orientations slowed the whole batch
+// from 49 to 57 ms).
+constexpr double kRecursionMargin = 1.02;
 
 }  // namespace
 
@@ -132,6 +83,13 @@ fm_status fm_autotune_estimate(fm_dtype dtype, fm_mode mode, const char* alg, in
   *us = estimate_us(dtype, mode, a, m, k, n, num_steps);
   return FM_OK;
 }
+
+fm_status fm_autotune_load(const char* path) {
+  clear_error();
+  return cost_table_load(path);
+}
+
+const char* fm_autotune_table(void) { return cost_table().source.c_str(); }
 
 fm_status fm_batch_create(fm_dtype dtype, fm_mode mode, const fm_problem* problems, int count,
                           const char* const* candidates, int num_candidates, int max_steps,
@@ -187,7 +145,11 @@ fm_status fm_batch_create(fm_dtype dtype, fm_mode mode, const fm_problem* proble
       const int smin = steps >= 0 ? steps : 0, smax = steps >= 0 ? steps : max_steps;
       for (const FmAlgDesc* a : pool)
         for (int s = smin; s <= smax; ++s) {
-          const double t = estimate_us(dtype, mode, s == 0 ? nullptr : a, q.m, q.k, q.n, s);
+          double t = estimate_us(dtype, mode, s == 0 ? nullptr : a, q.m, q.k, q.n, s);
+          // a recursion must beat the classical product by a clear margin:
+          // in a batch its odd-shaped leaves share the grouped launch with
+          // every other problem's, which the per-problem model does not see
+          if (s > 0) t *= kRecursionMargin;
           if (t < best) {
             best = t;
             best_alg = s == 0 ? nullptr : a;

@@ -124,15 +124,21 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   const int lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
 
-  // Interior tiles of products whose A and B columns start 16-byte aligned
-  // (checked per product: one misaligned product of a grouped launch does
-  // not slow the others; args.vec covers the base pointers) take
-  // unpredicated 16-byte loads for every full K slice; everything else
-  // takes the predicated element loads.
+  // Tiles of products whose A and B columns start 16-byte aligned (checked
+  // per product: one misaligned product of a grouped launch does not slow
+  // the others; args.vec covers the base pointers) take 16-byte loads for
+  // every full K slice: unpredicated inside the product, and on its right
+  // and bottom edges predicated per 16-byte vector (a B vector is VL
+  // consecutive k of one column, an A vector VL consecutive rows, which lie
+  // wholly inside or wholly outside the product when VL divides M).  Tiles
+  // of other products, and partial K slices, take the predicated element
+  // loads.  Out-of-range elements read as zero either way, so the MFMA
+  // sequence and the results are the same.
   static_assert(!FULL || CAN_VEC, "full-tile variant needs the vector load mapping");
   const bool d_aligned = d.a.off % VL == 0 && d.a.ld % VL == 0 && d.b.off % VL == 0 && d.b.ld % VL == 0;
+  const bool edge_free = FULL || ((m0 + BM <= M) && (n0 + BN <= N));
   const bool interior =
-      FULL || (CAN_VEC && args.vec && d_aligned && (m0 + BM <= M) && (n0 + BN <= N));
+      FULL || (CAN_VEC && args.vec && d_aligned && (m0 + BM <= M || M % VL == 0));
 
   // element-load coordinates
   const int a_row = tid % BM, a_k = tid / BM;
@@ -154,19 +160,38 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
 
   T ra[A_PER], rb[B_PER];
 
+  const bool av_in = m0 + av_row < M;  // this lane's A vector rows lie inside the product
   auto gload_vec = [&](int k0) {
     if constexpr (!CAN_VEC) return;
+    if (edge_free) {
 #pragma unroll
-    for (int i = 0; i < AV_PER; ++i) {
-      const vec_t v = *reinterpret_cast<const vec_t*>(av_ptr + (int64_t)(k0 + i * AV_KSTEP) * lda);
+      for (int i = 0; i < AV_PER; ++i) {
+        const vec_t v = *reinterpret_cast<const vec_t*>(av_ptr + (int64_t)(k0 + i * AV_KSTEP) * lda);
 #pragma unroll
-      for (int e = 0; e < VL; ++e) ra[i * VL + e] = v[e];
-    }
+        for (int e = 0; e < VL; ++e) ra[i * VL + e] = v[e];
+      }
 #pragma unroll
-    for (int i = 0; i < BV_PER; ++i) {
-      const vec_t v = *reinterpret_cast<const vec_t*>(bv_ptr + k0 + (int64_t)(i * BV_NSTEP) * ldb);
+      for (int i = 0; i < BV_PER; ++i) {
+        const vec_t v = *reinterpret_cast<const vec_t*>(bv_ptr + k0 + (int64_t)(i * BV_NSTEP) * ldb);
 #pragma unroll
-      for (int e = 0; e < VL; ++e) rb[i * VL + e] = v[e];
+        for (int e = 0; e < VL; ++e) rb[i * VL + e] = v[e];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < AV_PER; ++i) {
+        vec_t v = vec_t{};
+        if (av_in) v = *reinterpret_cast<const vec_t*>(av_ptr + (int64_t)(k0 + i * AV_KSTEP) * lda);
+#pragma unroll
+        for (int e = 0; e < VL; ++e) ra[i * VL + e] = v[e];
+      }
+#pragma unroll
+      for (int i = 0; i < BV_PER; ++i) {
+        vec_t v = vec_t{};
+        if (n0 + bv_col + i * BV_NSTEP < N)
+          v = *reinterpret_cast<const vec_t*>(bv_ptr + k0 + (int64_t)(i * BV_NSTEP) * ldb);
+#pragma unroll
+        for (int e = 0; e < VL; ++e) rb[i * VL + e] = v[e];
+      }
     }
   };
   auto sstore_vec = [&](int buf) {

@@ -463,8 +463,13 @@ bool carve_full(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods, std::vec
   const int bk = fm_tile_bk(c->cfg_full), vl = P.dtype == FM_F64 ? 2 : 4;
   std::vector<FmGemmDesc> core, st;
   for (const FmGemmDesc& d : prods) {
-    if (d.K % bk || d.K == 0 || d.a.off % vl || d.a.ld % vl || d.b.off % vl || d.b.ld % vl)
-      return false;
+    if (d.K % bk || d.K == 0 || d.a.off % vl || d.a.ld % vl || d.b.off % vl || d.b.ld % vl) {
+      // a product the whole-tile kernel cannot take (ragged K, unaligned
+      // columns) runs whole in the second launch, so that it does not send
+      // every other product of a grouped launch to the edge kernel
+      st.push_back(d);
+      continue;
+    }
     const int64_t Mf = d.M / bm * bm, Nf = d.N / bn * bn;
     if (Mf > 0 && Nf > 0) core.push_back(gsub(d, 0, 0, Mf, Nf));
     if (d.N > Nf) st.push_back(gsub(d, 0, Nf, d.M, d.N - Nf));          // right strip, all rows
@@ -776,6 +781,101 @@ fm_status check_stage_size(const Launch& L) {
 }
 }  // namespace
 
+namespace {
+// Output slabs per node of a stage launch's layout (see FmNodeView).
+void stage_slabs(const Launch& L, int* nout, int* nview) {
+  const FmAlgDesc* g = L.alg;
+  *nview = 0;
+  switch (L.kind) {
+    case LK_S: *nout = g->ns; break;
+    case LK_T: *nout = g->nt; break;
+    case LK_W: *nout = g->R; break;
+    case LK_S2: *nout = g->R * g->ns; *nview = g->R * g->a_nviews; break;
+    case LK_T2: *nout = g->R * g->nt; *nview = g->R * g->b_nviews; break;
+    case LK_W2: *nout = g->R * g->R; break;
+    default: *nout = 0; break;
+  }
+}
+
+// One launch for stage launches of the same kind and algorithm from several
+// problems (their block shapes may differ): per-node descriptors, the
+// largest shape for the grid, 16-byte accesses only if every node allows
+// them.  The kernels compute every node exactly as its own launch would.
+Launch group_stages(std::vector<Launch>& ls, fm_dtype dt) {
+  Launch G;
+  const Launch& f = ls[0];
+  G.kind = f.kind;
+  G.alg = f.alg;
+  G.stage = f.stage;
+  G.st = f.st;
+  G.st.rows = G.st.cols = G.st.rows2 = G.st.cols2 = 0;
+  int vec = dt == FM_F64 ? 2 : 4;
+  for (const Launch& L : ls) {
+    int nout, nview;
+    stage_slabs(L, &nout, &nview);
+    vec = std::min(vec, L.vec_ok);
+    G.st.rows = std::max(G.st.rows, L.st.rows);
+    G.st.cols = std::max(G.st.cols, L.st.cols);
+    G.st.rows2 = std::max(G.st.rows2, L.st.rows2);
+    G.st.cols2 = std::max(G.st.cols2, L.st.cols2);
+    for (size_t q = 0; q < L.ops.size(); ++q) {
+      FmStageNode n;
+      memset(&n, 0, sizeof(n));
+      n.op = L.ops[q];
+      n.rows = L.st.rows;
+      n.cols = L.st.cols;
+      n.rows2 = L.st.rows2;
+      n.cols2 = L.st.cols2;
+      n.out_off = L.st.out_off + (int64_t)q * nout * L.st.out_slab;
+      n.out_ld = L.st.out_ld;
+      n.out_slab = L.st.out_slab;
+      n.view_off = L.st.view_off + (int64_t)q * nview * L.st.out_slab;
+      n.x = L.st.x;
+      n.beta = L.st.beta;
+      G.snodes.push_back(n);
+    }
+  }
+  G.st.nodes = (int32_t)G.snodes.size();
+  G.vec_ok = vec;
+  return G;
+}
+
+// Merge the stage launches of several problems: launches at the same
+// position of their problems' sequences with the same kind, algorithm and
+// (for W) beta class become one grouped launch; positions keep their order,
+// so every problem's own stage order is kept.
+void group_sequence(std::vector<std::vector<Launch>*> seqs, std::vector<Launch>& out, fm_dtype dt) {
+  size_t depth = 0;
+  for (auto* sq : seqs) depth = std::max(depth, sq->size());
+  for (size_t i = 0; i < depth; ++i) {
+    std::vector<std::pair<std::tuple<int, const FmAlgDesc*, int>, std::vector<Launch>>> groups;
+    for (auto* sq : seqs) {
+      if (i >= sq->size() || (*sq)[i].kind == LK_NONE) continue;
+      Launch& L = (*sq)[i];
+      const bool stage = L.kind == LK_S || L.kind == LK_T || L.kind == LK_W || L.kind == LK_S2 ||
+                         L.kind == LK_T2 || L.kind == LK_W2;
+      if (!stage) {  // a depth-first child's leaf launch: kept as it is
+        out.push_back(std::move(L));
+        continue;
+      }
+      const auto key = std::make_tuple((int)L.kind, L.alg, L.st.beta != 0.0 ? 1 : 0);
+      auto it = std::find_if(groups.begin(), groups.end(), [&](const auto& g) { return g.first == key; });
+      if (it == groups.end()) {
+        groups.emplace_back(key, std::vector<Launch>());
+        it = groups.end() - 1;
+      }
+      it->second.push_back(std::move(L));
+    }
+    for (auto& g : groups) {
+      if (g.second.size() == 1)
+        out.push_back(std::move(g.second[0]));
+      else
+        out.push_back(group_stages(g.second, dt));
+    }
+  }
+}
+}  // namespace
+
 fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int persistent_grid) {
   for (SubPlan& s : subs) {
     for (const Launch& L : s.pre) {
@@ -787,8 +887,15 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
       if (st != FM_OK) return st;
     }
   }
-  for (SubPlan& s : subs)
-    for (Launch& L : s.pre) P.launches.push_back(std::move(L));
+  if (subs.size() > 1) {
+    // several problems (a batch): one launch per stage position and kind
+    std::vector<std::vector<Launch>*> seqs;
+    for (SubPlan& s : subs) seqs.push_back(&s.pre);
+    group_sequence(seqs, P.launches, P.dtype);
+  } else {
+    for (SubPlan& s : subs)
+      for (Launch& L : s.pre) P.launches.push_back(std::move(L));
+  }
   std::vector<FmGemmDesc> leaves;
   for (SubPlan& s : subs) leaves.insert(leaves.end(), s.leaves.begin(), s.leaves.end());
   size_t wsb = (size_t)ws_elems * elem_size(P.dtype);
@@ -836,10 +943,23 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
   for (const SubPlan& s : subs) maxd = std::max(maxd, s.depth);
   for (int l = maxd - 1; l >= 0; --l) {
     std::vector<FmGemmDesc> peel;
+    std::vector<Launch> wl;
     for (SubPlan& s : subs) {
       if (s.depth <= l) continue;
-      if (s.w[l].kind != LK_NONE) P.launches.push_back(std::move(s.w[l]));
+      if (s.w[l].kind != LK_NONE) wl.push_back(std::move(s.w[l]));
       peel.insert(peel.end(), s.peels[l].begin(), s.peels[l].end());
+    }
+    if (wl.size() > 1) {
+      // this level's W stages of every problem: grouped by kind, algorithm, beta class
+      std::vector<std::vector<Launch>> one(wl.size());
+      std::vector<std::vector<Launch>*> seqs;
+      for (size_t i = 0; i < wl.size(); ++i) {
+        one[i].push_back(std::move(wl[i]));
+        seqs.push_back(&one[i]);
+      }
+      group_sequence(seqs, P.launches, P.dtype);
+    } else {
+      for (Launch& L : wl) P.launches.push_back(std::move(L));
     }
     // peeling products stay on the plain kernel in OZAKI mode (FP64) and
     // take the split kernel in SPLIT_BF16 mode
@@ -871,7 +991,9 @@ void Plan::pack(std::vector<char>& buf) {
     else if (L.kind == LK_SPLIT) {
       L.dev_off = put(L.sdescs.data(), L.sdescs.size() * sizeof(FmSplitDesc));
       L.dev_off2 = put(L.spreps.data(), L.spreps.size() * sizeof(FmSplitPrep));
-    } else
+    } else if (!L.snodes.empty())
+      L.dev_off = put(L.snodes.data(), L.snodes.size() * sizeof(FmStageNode));
+    else
       L.dev_off = put(L.ops.data(), L.ops.size() * sizeof(FmOp));
   }
   ext_off = put(ext.data(), ext.size() * sizeof(void*));
@@ -889,8 +1011,13 @@ void Plan::bind(void* dev) {
     else if (L.kind == LK_SPLIT) {
       L.sp.descs = reinterpret_cast<const FmSplitDesc*>(p);
       L.sp.preps = reinterpret_cast<const FmSplitPrep*>((char*)dev + L.dev_off2);
-    } else
+    } else if (!L.snodes.empty()) {
+      L.st.node = reinterpret_cast<const FmStageNode*>(p);
+      L.st.ops = nullptr;
+    } else {
       L.st.ops = reinterpret_cast<const FmOp*>(p);
+      L.st.node = nullptr;
+    }
   }
 }
 

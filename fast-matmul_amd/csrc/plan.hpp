@@ -35,6 +35,7 @@ struct Launch {
   int vec_ok = 1;
   int oz_slices = 0, oz_rows = 0, oz_cols = 0;
   std::vector<FmOp> ops;
+  std::vector<FmStageNode> snodes;  // grouped stage launch: per-node descriptors
   std::vector<FmGemmDesc> gdescs;
   std::vector<FmOzakiDesc> odescs;
   std::vector<FmSplitDesc> sdescs;

@@ -72,13 +72,14 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage(FmStageArgs a) {
   constexpr int NOUT = SIDE == 0 ? Cmb::NS : Cmb::NT;
   constexpr int NOUT_ARR = NOUT > 0 ? NOUT : 1;
   const int q = blockIdx.y;
-  const FmOp op = a.ops[q];
+  const FmNodeView nv = fm_node_view(a, q, NOUT, 0);
+  const FmOp op = nv.op;
   const T* src = reinterpret_cast<const T*>(fm_base(a.bases, op.role)) + op.off;
-  T* dst = reinterpret_cast<T*>(a.bases.p[FM_ROLE_WS]) + a.out_off +
-           (int64_t)q * NOUT * a.out_slab;
-  const uint32_t rv = (uint32_t)a.rows / VEC;
-  const uint32_t total = rv * (uint32_t)a.cols;
-  const int64_t ld = op.ld;
+  T* dst = reinterpret_cast<T*>(a.bases.p[FM_ROLE_WS]) + nv.out;
+  const uint32_t rv = (uint32_t)nv.rows / VEC;
+  const uint32_t total = rv * (uint32_t)nv.cols;
+  const int64_t ld = op.ld, rows = nv.rows, cols = nv.cols, out_ld = nv.out_ld, slab = nv.out_slab;
+  const double x = nv.x;
   for (uint32_t e = blockIdx.x * FM_STAGE_THREADS + threadIdx.x; e < total;
        e += gridDim.x * FM_STAGE_THREADS) {
     const uint32_t j = e / rv;
@@ -87,7 +88,7 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage(FmStageArgs a) {
 #pragma unroll
     for (int p = 0; p < NB; ++p) {
       const int bi = p / BC, bj = p % BC;
-      v[p] = fm_load<T, VEC>(src + (int64_t)bi * a.rows + i + ((int64_t)bj * a.cols + j) * ld);
+      v[p] = fm_load<T, VEC>(src + (int64_t)bi * rows + i + ((int64_t)bj * cols + j) * ld);
     }
     FmVec<T, VEC> o[NOUT_ARR];
 #pragma unroll
@@ -97,15 +98,15 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage(FmStageArgs a) {
       for (int p = 0; p < NB; ++p) in[p] = v[p].v[l];
       T out[NOUT_ARR];
       if constexpr (SIDE == 0)
-        Cmb::s_combine(in, out, a.x);
+        Cmb::s_combine(in, out, x);
       else
-        Cmb::t_combine(in, out, a.x);
+        Cmb::t_combine(in, out, x);
 #pragma unroll
       for (int t = 0; t < NOUT; ++t) o[t].v[l] = out[t];
     }
 #pragma unroll
     for (int t = 0; t < NOUT; ++t)
-      fm_store<T, VEC>(dst + (int64_t)t * a.out_slab + i + (int64_t)j * a.out_ld, o[t]);
+      fm_store<T, VEC>(dst + (int64_t)t * slab + i + (int64_t)j * out_ld, o[t]);
   }
 }
 
@@ -116,14 +117,15 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage(FmStageArgs a) {
   constexpr int R = Cmb::R;
   constexpr int MN = Cmb::m * Cmb::n;
   const int q = blockIdx.y;
-  const FmOp op = a.ops[q];
+  const FmNodeView nv = fm_node_view(a, q, R, 0);
+  const FmOp op = nv.op;
   T* c = reinterpret_cast<T*>(fm_base(a.bases, op.role)) + op.off;
-  const T* mb = reinterpret_cast<const T*>(a.bases.p[FM_ROLE_WS]) + a.out_off +
-                (int64_t)q * R * a.out_slab;
-  const uint32_t rv = (uint32_t)a.rows / VEC;
-  const uint32_t total = rv * (uint32_t)a.cols;
-  const int64_t ld = op.ld;
-  const T beta = (T)a.beta;
+  const T* mb = reinterpret_cast<const T*>(a.bases.p[FM_ROLE_WS]) + nv.out;
+  const uint32_t rv = (uint32_t)nv.rows / VEC;
+  const uint32_t total = rv * (uint32_t)nv.cols;
+  const int64_t ld = op.ld, rows = nv.rows, cols = nv.cols, out_ld = nv.out_ld, slab = nv.out_slab;
+  const T beta = (T)nv.beta;
+  const double x = nv.x;
   for (uint32_t e = blockIdx.x * FM_STAGE_THREADS + threadIdx.x; e < total;
        e += gridDim.x * FM_STAGE_THREADS) {
     const uint32_t j = e / rv;
@@ -131,7 +133,7 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage(FmStageArgs a) {
     FmVec<T, VEC> mv[R];
 #pragma unroll
     for (int r = 0; r < R; ++r)
-      mv[r] = fm_load<T, VEC>(mb + (int64_t)r * a.out_slab + i + (int64_t)j * a.out_ld);
+      mv[r] = fm_load<T, VEC>(mb + (int64_t)r * slab + i + (int64_t)j * out_ld);
     FmVec<T, VEC> o[MN];
 #pragma unroll
     for (int l = 0; l < VEC; ++l) {
@@ -139,14 +141,14 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage(FmStageArgs a) {
 #pragma unroll
       for (int r = 0; r < R; ++r) in[r] = mv[r].v[l];
       T out[MN];
-      Cmb::w_combine(in, out, a.x);
+      Cmb::w_combine(in, out, x);
 #pragma unroll
       for (int t = 0; t < MN; ++t) o[t].v[l] = out[t];
     }
 #pragma unroll
     for (int t = 0; t < MN; ++t) {
       const int bi = t / Cmb::n, bj = t % Cmb::n;
-      T* p = c + (int64_t)bi * a.rows + i + ((int64_t)bj * a.cols + j) * ld;
+      T* p = c + (int64_t)bi * rows + i + ((int64_t)bj * cols + j) * ld;
       if constexpr (BETA) {
         FmVec<T, VEC> old = fm_load<T, VEC>(p);
 #pragma unroll
@@ -186,15 +188,17 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2(FmStageArgs a) 
   constexpr int NOUT_ARR = NOUT > 0 ? NOUT : 1;
   constexpr int NV = SIDE == 0 ? Cmb::a_nviews : Cmb::b_nviews;
   const int q = blockIdx.y;
-  const FmOp op = a.ops[q];
+  const FmNodeView nv = fm_node_view(a, q, R * NOUT, R * NV);
+  const FmOp op = nv.op;
   const T* src = reinterpret_cast<const T*>(fm_base(a.bases, op.role)) + op.off;
   T* ws = reinterpret_cast<T*>(a.bases.p[FM_ROLE_WS]);
-  T* dst = ws + a.out_off + (int64_t)q * R * NOUT * a.out_slab;
-  T* vdst = ws + a.view_off + (int64_t)q * R * NV * a.out_slab;
-  const uint32_t rows = (uint32_t)a.rows2;
-  const uint32_t total = rows * (uint32_t)a.cols2;
-  const int64_t ld = op.ld;
-  const double x = a.x;
+  T* dst = ws + nv.out;
+  T* vdst = ws + nv.view;
+  const uint32_t rows = (uint32_t)nv.rows2;
+  const uint32_t total = rows * (uint32_t)nv.cols2;
+  const int64_t ld = op.ld, brows = nv.rows, bcols = nv.cols, brows2 = nv.rows2,
+                bcols2 = nv.cols2, slab = nv.out_slab;
+  const double x = nv.x;
   for (uint32_t e = blockIdx.x * FM_STAGE_THREADS + threadIdx.x; e < total;
        e += gridDim.x * FM_STAGE_THREADS) {
     const uint32_t j = e / rows;
@@ -204,11 +208,11 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2(FmStageArgs a) 
     for (int b = 0; b < NB; ++b)
 #pragma unroll
       for (int sb = 0; sb < NB; ++sb) {
-        const int64_t r0 = (int64_t)(b / BC) * a.rows + (int64_t)(sb / BC) * a.rows2 + i;
-        const int64_t c0 = (int64_t)(b % BC) * a.cols + (int64_t)(sb % BC) * a.cols2 + j;
+        const int64_t r0 = (int64_t)(b / BC) * brows + (int64_t)(sb / BC) * brows2 + i;
+        const int64_t c0 = (int64_t)(b % BC) * bcols + (int64_t)(sb % BC) * bcols2 + j;
         v[b][sb] = src[r0 + c0 * ld];
       }
-    const int64_t opos = i + (int64_t)j * a.out_ld;
+    const int64_t opos = i + (int64_t)j * nv.out_ld;
     auto product = [&](auto rc) {
       constexpr int r = decltype(rc)::value;
       constexpr int kind = SIDE == 0 ? Cmb::a_kind_c[r] : Cmb::b_kind_c[r];
@@ -234,12 +238,12 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2(FmStageArgs a) 
       else
         Cmb::t_combine(par, out, x);
 #pragma unroll
-      for (int t = 0; t < NOUT; ++t) dst[((int64_t)r * NOUT + t) * a.out_slab + opos] = out[t];
+      for (int t = 0; t < NOUT; ++t) dst[((int64_t)r * NOUT + t) * slab + opos] = out[t];
       if constexpr (kind == 1) {
 #pragma unroll
         for (int sb = 0; sb < NB; ++sb) {
           const int slot = SIDE == 0 ? Cmb::a_view_slot[sb] : Cmb::b_view_slot[sb];
-          if (slot >= 0) vdst[((int64_t)r * NV + slot) * a.out_slab + opos] = par[sb];
+          if (slot >= 0) vdst[((int64_t)r * NV + slot) * slab + opos] = par[sb];
         }
       }
     };
@@ -274,33 +278,36 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage2(FmStageArgs a) {
   static_assert(P > 0, "level-(l+1) products do not fit in LDS");
   __shared__ T m1[R][MN][P];
   const int q = blockIdx.y;
-  const int rows2 = a.rows2;
+  const FmNodeView nv = fm_node_view(a, q, R * R, 0);
+  const int rows2 = nv.rows2;
   const int chunks = (rows2 + P - 1) / P;
+  if ((int)blockIdx.x >= chunks * nv.cols2) return;  // grouped launch: a smaller node
   const int j = blockIdx.x / chunks;           // column within the sub-block
   const int i0 = (blockIdx.x - j * chunks) * P;  // first row
-  const T* mb = reinterpret_cast<const T*>(a.bases.p[FM_ROLE_WS]) + a.out_off +
-                (int64_t)q * R * R * a.out_slab;
-  const int64_t col_off = (int64_t)j * a.out_ld;
+  const T* mb = reinterpret_cast<const T*>(a.bases.p[FM_ROLE_WS]) + nv.out;
+  const int64_t col_off = (int64_t)j * nv.out_ld;
+  const int64_t slab = nv.out_slab;
+  const double x = nv.x;
   // 1. level-(l+1) W, one (row, r) item per thread and pass
   for (int item = threadIdx.x; item < P * R; item += FM_STAGE_THREADS) {
     const int p = item % P, r = item / P;
     const int i = i0 + p;
     if (i >= rows2) continue;
     T mv[R];
-    const T* src = mb + (int64_t)r * R * a.out_slab + i + col_off;
+    const T* src = mb + (int64_t)r * R * slab + i + col_off;
 #pragma unroll
-    for (int r2 = 0; r2 < R; ++r2) mv[r2] = src[(int64_t)r2 * a.out_slab];
+    for (int r2 = 0; r2 < R; ++r2) mv[r2] = src[(int64_t)r2 * slab];
     T out[MN];
-    Cmb::w_combine(mv, out, a.x);
+    Cmb::w_combine(mv, out, x);
 #pragma unroll
     for (int t = 0; t < MN; ++t) m1[r][t][p] = out[t];
   }
   __syncthreads();
   // 2. level-l W, one (row, sub-block t) item per thread and pass
-  const FmOp op = a.ops[q];
+  const FmOp op = nv.op;
   T* c = reinterpret_cast<T*>(fm_base(a.bases, op.role)) + op.off;
   const int64_t ld = op.ld;
-  const T beta = (T)a.beta;
+  const T beta = (T)nv.beta;
   for (int item = threadIdx.x; item < P * MN; item += FM_STAGE_THREADS) {
     const int p = item % P, t = item / P;
     const int i = i0 + p;
@@ -309,13 +316,13 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage2(FmStageArgs a) {
 #pragma unroll
     for (int r = 0; r < R; ++r) mv[r] = m1[r][t][p];
     T out[MN];
-    Cmb::w_combine(mv, out, a.x);
+    Cmb::w_combine(mv, out, x);
     const int ti = t / Cmb::n, tj = t % Cmb::n;
-    const int64_t r0 = (int64_t)ti * rows2 + i, c0 = (int64_t)tj * a.cols2 + j;
+    const int64_t r0 = (int64_t)ti * rows2 + i, c0 = (int64_t)tj * nv.cols2 + j;
 #pragma unroll
     for (int b = 0; b < MN; ++b) {
       const int bi = b / Cmb::n, bj = b % Cmb::n;
-      T* dst = c + (int64_t)bi * a.rows + r0 + ((int64_t)bj * a.cols + c0) * ld;
+      T* dst = c + (int64_t)bi * nv.rows + r0 + ((int64_t)bj * nv.cols + c0) * ld;
       T v = out[b];
       if constexpr (BETA) v = fm_add(v, fm_mul(beta, *dst));
       *dst = v;

@@ -410,6 +410,16 @@ class Batch:
             self._h = None
 
 
+def autotune_load(path: str):
+    """Use the cost-model table at ``path`` (as written by fastmm.calibrate)."""
+    check(LIB.fm_autotune_load(path.encode()))
+
+
+def autotune_table() -> str:
+    """Path of the cost-model table in use ("built-in" without one)."""
+    return LIB.fm_autotune_table().decode()
+
+
 def autotune_estimate(alg: Optional[str], m: int, k: int, n: int, num_steps: int,
                       dtype=torch.float32, mode: str = "plain") -> float:
     """The autotuner's cost-model estimate in microseconds."""

@@ -34,7 +34,7 @@ EXPORTS = (
     "fm_set_stage_fusion", "fm_set_schedule", "fm_plan_query", "fm_fastmatmul_scaled",
     "fm_options_default", "fm_fastmatmul_ex", "fm_plan_create", "fm_plan_run",
     "fm_plan_workspace", "fm_plan_destroy", "fm_scaled_workspace_query", "fm_release_workspace",
-    "fm_fastmatmul_noalloc", "fm_probe_mfma_peak",
+    "fm_fastmatmul_noalloc", "fm_probe_mfma_peak", "fm_autotune_load", "fm_autotune_table",
 )
 
 # the fixed scratch array of fm_fastmatmul_noalloc (FM_NOALLOC_SCRATCH_BYTES)
@@ -99,6 +99,10 @@ def _load():
     lib.fm_fastmatmul_noalloc.argtypes = [ctypes.c_char_p, c_int, c_int, c_int, c_int, c_int,
                                           vp, i64, vp, i64, vp, i64, c_int, c_double, c_double,
                                           c_double, vp, ctypes.POINTER(ctypes.c_float)]
+    lib.fm_autotune_load.restype = c_int
+    lib.fm_autotune_load.argtypes = [ctypes.c_char_p]
+    lib.fm_autotune_table.restype = ctypes.c_char_p
+    lib.fm_autotune_table.argtypes = []
     lib.fm_probe_mfma_peak.restype = c_int
     lib.fm_probe_mfma_peak.argtypes = [c_int, c_int, ctypes.POINTER(c_double)]
     lib.fm_options_default.restype = None

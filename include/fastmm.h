@@ -195,9 +195,16 @@ size_t fm_batch_workspace_bytes(fm_batch batch);
 void fm_batch_destroy(fm_batch batch);
 
 /* The autotuner's estimate (microseconds) for one multiply; alg == NULL or
- * num_steps == 0 prices the classical product. */
+ * num_steps == 0 prices the classical product.  The cost model's rates (leaf
+ * GEMM TFLOP/s against leaf size per dtype and mode, stage bandwidth, launch
+ * and per-level costs) come from a text table measured on the GPU by
+ * `python -m fastmm.calibrate`: FM_TUNING_FILE, else tuning/mi355x.txt beside
+ * the library, else built-in values.  fm_autotune_load replaces the table;
+ * fm_autotune_table names the one in use. */
 fm_status fm_autotune_estimate(fm_dtype dtype, fm_mode mode, const char* alg, int m, int k, int n,
                                int num_steps, double* us);
+fm_status fm_autotune_load(const char* path);
+const char* fm_autotune_table(void);
 
 /* Registry (algorithms/all_algorithms.hpp:52-90, 247-326). */
 int fm_num_algorithms(void);

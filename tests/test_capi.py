@@ -167,3 +167,35 @@ def test_scaled_workspace_query():
     base = al(plan) + al(8 * m * k) + al(8 * k * n) + al(8 * m) + al(8 * n) + 2 * al(8 * L) + 2 * al(8 * L)
     assert fastmm.scaled_workspace_bytes("strassen", m, k, n, 2) == base
     assert fastmm.scaled_workspace_bytes("strassen", m, k, n, 2, beta_nonzero=True) == base + al(8 * m * n)
+
+
+def test_cost_table_load_and_interpolation(tmp_path):
+    """The autotuner's cost table: the packaged, GPU-measured one is in use;
+    another file replaces it; leaf rates interpolate in log size; a
+    malformed file is refused and leaves the table unchanged."""
+    import fastmm
+    import torch
+    packaged = fastmm.autotune_table()
+    assert packaged.endswith(os.path.join("tuning", "mi355x.txt")), packaged
+    f32 = torch.float32
+    try:
+        t = tmp_path / "t.txt"
+        t.write_text("leaf f32 plain 512 50\nleaf f32 plain 2048 200\nstage_gbps f32 1000\n"
+                     "stage_gbps f64 1000\nlaunch_us 0\nlevel_us f32 0\nlevel_us f64 0\n")
+        fastmm.autotune_load(str(t))
+        assert fastmm.autotune_table() == str(t)
+        # 2 * 512^3 at 50 TFLOP/s, 2 * 2048^3 at 200, 1024 halfway (125) in log size
+        assert abs(fastmm.autotune_estimate(None, 512, 512, 512, 0, f32) - 2 * 512 ** 3 / 50e6) < 1e-6
+        assert abs(fastmm.autotune_estimate(None, 2048, 2048, 2048, 0, f32) - 2 * 2048 ** 3 / 200e6) < 1e-6
+        assert abs(fastmm.autotune_estimate(None, 1024, 1024, 1024, 0, f32) - 2 * 1024 ** 3 / 125e6) < 1e-6
+        # one Strassen level at 2048: stage bytes at 1 TB/s plus 7 leaves of 1024^3
+        s_bytes = 4 * (1024 ** 2 * (4 + 5) * 2 + 1024 ** 2 * (7 + 4))
+        want = s_bytes / 1e6 + 7 * 2 * 1024 ** 3 / 125e6  # bytes / (1000 GB/s) in us
+        assert abs(fastmm.autotune_estimate("strassen", 2048, 2048, 2048, 1, f32) - want) < 1e-3
+        bad = tmp_path / "bad.txt"
+        bad.write_text("leaf f32 plain 512\n")
+        with pytest.raises(ValueError):
+            fastmm.autotune_load(str(bad))
+        assert fastmm.autotune_table() == str(t)
+    finally:
+        fastmm.autotune_load(packaged)

@@ -120,3 +120,88 @@ def test_batch_split_planes_match_single_calls(fm):
         for i, (p, w) in enumerate(zip(probs, want)):
             got = p["C"].cpu().numpy()
             assert np.array_equal(got.view(np.uint32), w.view(np.uint32)), (rep, i, b.choice(i))
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_batch_grouped_stages_match_single_calls(fm, dtype):
+    """The S/T/W stages of every problem that shares an algorithm and a
+    recursion position run as one grouped launch (per-node shapes): each
+    problem still equals its own single call bit for bit, with peeling,
+    two-level stages, alpha/beta and three depths, and the batch issues far
+    fewer stage launches than its problems have levels."""
+    shapes = [(512 + 64 * i, 576 + 128 * (i % 3), 640 + 64 * (i % 4)) for i in range(10)]
+    probs, host = _problems(fm, shapes, dtype, "u11", 6)
+    algs = [("strassen", 2), ("strassen", 1), ("grey423_20_144", 2), ("strassen", 2),
+            ("grey423_20_144", 1), ("smirnov333_23_139", 2), ("smirnov333_23_139", 1),
+            ("strassen", 3), ("grey423_20_144", 2), ("strassen", 2)]
+    for i, (p, (a, st)) in enumerate(zip(probs, algs)):
+        p.update(alg=a, num_steps=st, alpha=1.0 + 0.25 * (i % 3), beta=(-0.5 if i % 2 else 0.0))
+    C0s = [p["C"].clone() for p in probs]
+    want = []
+    for p, C0 in zip(probs, C0s):
+        C = C0.clone()
+        fm.fastmatmul(p["alg"], p["A"], p["B"], C, p["num_steps"], alpha=p["alpha"], beta=p["beta"])
+        want.append(C)
+    b = fm.Batch(probs, graph=False)
+    fm.profile_read()
+    fm.profile_enable(True)
+    try:
+        b.run()
+        torch.cuda.synchronize()
+    finally:
+        fm.profile_enable(False)
+    prof = fm.profile_read()
+    for i, (p, w) in enumerate(zip(probs, want)):
+        assert torch.equal(p["C"], w), (i, p["alg"], p["num_steps"])
+    alone = 0  # stage launches the problems issue as single calls
+    for p, (a, st) in zip(probs, algs):
+        m, k = p["A"].shape
+        info = fm.plan_info(a, m, k, p["B"].shape[1], st, dtype=dtype)
+        alone += sum(info["launches"][s] for s in ("S", "T", "W"))
+    grouped = prof["S"][1] + prof["T"][1] + prof["W"][1]
+    assert grouped < alone, (grouped, alone)
+    # problems of one algorithm, depth and beta class share every stage launch
+    same = [dict(p, alg="strassen", num_steps=2, beta=0.0) for p in probs]
+    b2 = fm.Batch(same, graph=False)
+    fm.profile_enable(True)
+    try:
+        b2.run()
+        torch.cuda.synchronize()
+    finally:
+        fm.profile_enable(False)
+    prof = fm.profile_read()
+    m, k = probs[0]["A"].shape
+    one = fm.plan_info("strassen", m, k, probs[0]["B"].shape[1], 2, dtype=dtype)["launches"]
+    assert all(prof[s][1] == one[s] for s in ("S", "T", "W")), (prof, one)
+
+
+def test_batch_full_workload_256_uneven(fm, oracle_mod):
+    """The full batched workload: 256 independent FP32 multiplies, M, K, N
+    each uniform on {512, 576, ..., 4096}, uniform(-1,1), autotuned,
+    captured in a hipGraph.  Every problem is checked on 32 sampled entries
+    against the quad-precision oracle: |C - AB| <= 64 (1 + depth) 2^-24
+    (|A||B|)_ij."""
+    rng = np.random.default_rng(0xC3)
+    vals = np.arange(512, 4097, 64)
+    shapes = [tuple(int(v) for v in rng.choice(vals, 3)) for _ in range(256)]
+    g = torch.Generator(device=DEV)
+    g.manual_seed(0xC3)
+    probs = []
+    for (m, k, n) in shapes:
+        A = fm.col_major(torch.rand((m, k), generator=g, device=DEV, dtype=torch.float32) * 2 - 1)
+        B = fm.col_major(torch.rand((k, n), generator=g, device=DEV, dtype=torch.float32) * 2 - 1)
+        probs.append({"A": A, "B": B, "C": fm.empty(m, n, torch.float32, DEV)})
+    b = fm.Batch(probs, graph=True)
+    b.run()
+    b.run()
+    torch.cuda.synchronize()
+    for i, (p, (m, k, n)) in enumerate(zip(probs, shapes)):
+        alg, st = b.choice(i)
+        rows = torch.from_numpy(rng.integers(0, m, 32)).to(DEV)
+        cols = torch.from_numpy(rng.integers(0, n, 32)).to(DEV)
+        Ar = np.asfortranarray(p["A"][rows].double().cpu().numpy())
+        Bc = np.asfortranarray(p["B"][:, cols].double().cpu().numpy())
+        e = np.arange(32)
+        q, absab = oracle_mod.quad_entries(Ar, Bc, e, e)
+        got = p["C"][rows, cols].double().cpu().numpy()
+        assert np.all(np.abs(got - q) <= 64 * (1 + st) * 2.0 ** -24 * absab), (i, alg, st)
