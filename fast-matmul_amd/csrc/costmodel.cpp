@@ -5,6 +5,7 @@
 //   stage_gbps <f64|f32> <GB/s>
 //   launch_us <us>
 //   level_us <f64|f32> <us>
+//   tile_k_us <f64|f32> <us per unit of K>
 #include "costmodel.hpp"
 
 #include <dlfcn.h>
@@ -56,13 +57,13 @@ bool parse(std::istream& in, CostTable& t, std::string* err) {
       const int d = dtype_index(dt), m = mode_index(mode);
       ok = ok && d >= 0 && m >= 0 && size > 0 && tf > 0;
       if (ok) r.leaf[d][m].emplace_back(size, tf);
-    } else if (key == "stage_gbps" || key == "level_us") {
+    } else if (key == "stage_gbps" || key == "level_us" || key == "tile_k_us") {
       std::string dt;
       double v = 0;
       ok = (bool)(ss >> dt >> v);
       const int d = dtype_index(dt);
-      ok = ok && d >= 0 && v >= 0 && (key == "level_us" || v > 0);
-      if (ok) (key == "stage_gbps" ? r.stage_gbps : r.level_us)[d] = v;
+      ok = ok && d >= 0 && v >= 0 && (key != "stage_gbps" || v > 0);
+      if (ok) (key == "stage_gbps" ? r.stage_gbps : key == "level_us" ? r.level_us : r.tile_k_us)[d] = v;
     } else if (key == "launch_us") {
       ok = (bool)(ss >> r.launch_us) && r.launch_us >= 0;
     } else {
@@ -155,7 +156,12 @@ double estimate_us(fm_dtype dt, fm_mode mode, const FmAlgDesc* alg, int64_t m, i
     const double size = std::cbrt((double)lm * lk * ln);
     return nodes * 2.0 * padded(lm) * padded(ln) * lk / (leaf_rate(t, dt, mode, size) * 1e6);
   };
-  if (!alg || steps == 0) return leaf_us(1.0, m, k, n) + t.launch_us;
+  // int8-digit leaves keep exact int32 sums only up to a K bound: a
+  // candidate whose leaves are deeper cannot run in that mode
+  auto too_deep = [&](int64_t lk) {
+    return mode == FM_MODE_OZAKI_INT8 && align_up(lk, 64) > fm_ozaki_max_kp(g_defaults.ozaki_slices);
+  };
+  if (!alg || steps == 0) return too_deep(k) ? INFINITY : leaf_us(1.0, m, k, n) + t.launch_us;
   double us = 0.0;
   int64_t rm = m, rk = k, rn = n;
   double nodes = 1.0;
@@ -168,17 +174,22 @@ double estimate_us(fm_dtype dt, fm_mode mode, const FmAlgDesc* alg, int64_t m, i
                          (blkA * (alg->m * alg->k + alg->ns) + blkB * (alg->k * alg->n + alg->nt) +
                           blkC * (alg->R + alg->m * alg->n));
     us += bytes / (t.stage_gbps[di] * 1e3);
-    // peeling products of this level's nodes, at the rate of their size
+    // peeling products of this level's nodes, at the rate of their size;
+    // their launch lasts at least as long as one tile takes over its K
     const int64_t erA = rm % alg->m, ecA = rk % alg->k, ecB = rn % alg->n;
-    if (ecA > 0) us += leaf_us(nodes, rm - erA, ecA, rn - ecB);
-    if (ecB > 0) us += leaf_us(nodes, rm, rk, ecB);
-    if (erA > 0) us += leaf_us(nodes, erA, rk, rn - ecB);
+    double peel = 0.0;
+    int64_t peel_k = 0;
+    if (ecA > 0) peel += leaf_us(nodes, rm - erA, ecA, rn - ecB), peel_k = std::max(peel_k, ecA);
+    if (ecB > 0) peel += leaf_us(nodes, rm, rk, ecB), peel_k = std::max(peel_k, rk);
+    if (erA > 0) peel += leaf_us(nodes, erA, rk, rn - ecB), peel_k = std::max(peel_k, rk);
+    us += std::max(peel, (double)peel_k * t.tile_k_us[di]);
     us += t.level_us[di];
     nodes *= alg->R;
     rm = bm;
     rk = bk;
     rn = bn;
   }
+  if (too_deep(rk)) return INFINITY;
   return us + leaf_us(nodes, rm, rk, rn) + t.launch_us;
 }
 

@@ -21,6 +21,8 @@ struct CostTable {
   double launch_us = 4.0;                   // one kernel launch
   double level_us[2] = {20.0, 20.0};        // one recursion level of a batched problem,
                                             // beyond its bytes and launches, per dtype
+  double tile_k_us[2] = {0.1, 0.1};         // one peeling tile's time per unit of K: a
+                                            // launch lasts at least its longest tile
   std::string source = "built-in";
 };
 

@@ -24,7 +24,7 @@ typedef float fm_f32x4 __attribute__((ext_vector_type(4)));
 // native 16-byte vector (register-resident in arrays, unlike the uint4 struct)
 typedef uint32_t fm_u32x4 __attribute__((ext_vector_type(4)));
 
-// round-to-nearest-even float -> bf16 (inputs are finite)
+// round-to-nearest-even float -> bf16 (finite inputs; fm_split3 handles the rest)
 __device__ __forceinline__ uint16_t fm_bf16_bits(float x) {
   uint32_t u = __float_as_uint(x);
   u += 0x7FFFu + ((u >> 16) & 1u);
@@ -34,16 +34,29 @@ __device__ __forceinline__ float fm_bf16_to_f32(uint16_t b) {
   return __uint_as_float((uint32_t)b << 16);
 }
 
-// split 8 values into three slice planes (8 bf16 each, packed in 4 uints)
+// split 8 values into three slice planes (8 bf16 each, packed in 4 uints).
+// Finite values whose rounding to bf16 would overflow (|x| at or above the
+// largest bf16 plus half an ulp) take the truncated value as the leading
+// slice instead, so their residuals stay finite (x - x0 is then exact: both
+// lie in the top binade).  Inf and NaN go through the leading slice with zero
+// residuals (a NaN keeps a set mantissa bit).
 __device__ __forceinline__ void fm_split3(const float (&x)[8], uint4 (&out)[3]) {
   uint32_t h[8], m[8], l[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    const uint16_t b0 = fm_bf16_bits(x[e]);
-    const float r1 = __fsub_rn(x[e], fm_bf16_to_f32(b0));
-    const uint16_t b1 = fm_bf16_bits(r1);
-    const float r2 = __fsub_rn(r1, fm_bf16_to_f32(b1));
-    const uint16_t b2 = fm_bf16_bits(r2);
+    const uint32_t u = __float_as_uint(x[e]);
+    uint16_t b0, b1, b2;
+    if ((u & 0x7F800000u) == 0x7F800000u) {
+      b0 = (uint16_t)((u >> 16) | ((u & 0x007FFFFFu) ? 0x0040u : 0u));
+      b1 = b2 = 0;
+    } else {
+      b0 = fm_bf16_bits(x[e]);
+      if ((b0 & 0x7F80u) == 0x7F80u) b0 = (uint16_t)(u >> 16);  // rounded to Inf: truncate
+      const float r1 = __fsub_rn(x[e], fm_bf16_to_f32(b0));
+      b1 = fm_bf16_bits(r1);
+      const float r2 = __fsub_rn(r1, fm_bf16_to_f32(b1));
+      b2 = fm_bf16_bits(r2);
+    }
     h[e] = b0;
     m[e] = b1;
     l[e] = b2;

@@ -454,20 +454,21 @@ FmGemmDesc gsub(const FmGemmDesc& d, int64_t i0, int64_t j0, int64_t m, int64_t 
 }
 
 bool carve_full(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods, std::vector<FmGemmDesc>& strips,
-                int* cfg_core, int* cfg_strips) {
+                std::vector<FmGemmDesc>& odd, int* cfg_core, int* cfg_strips, int* cfg_odd) {
   if (P.tile >= 0 || P.mode != FM_MODE_PLAIN) return false;
   const TileCand* c = find_cand(P.dtype, P.mode, cfg);
   if (!c || whole_tiles(P.dtype, c->cfg_full, prods)) return false;
   int bm, bn;
   fm_tile_dims(c->cfg_full, &bm, &bn);
   const int bk = fm_tile_bk(c->cfg_full), vl = P.dtype == FM_F64 ? 2 : 4;
-  std::vector<FmGemmDesc> core, st;
+  std::vector<FmGemmDesc> core, st, od;
   for (const FmGemmDesc& d : prods) {
     if (d.K % bk || d.K == 0 || d.a.off % vl || d.a.ld % vl || d.b.off % vl || d.b.ld % vl) {
       // a product the whole-tile kernel cannot take (ragged K, unaligned
-      // columns) runs whole in the second launch, so that it does not send
-      // every other product of a grouped launch to the edge kernel
-      st.push_back(d);
+      // columns) runs whole in a launch of its own, with its own tile, so
+      // that it neither sends every other product of a grouped launch to
+      // the edge kernel nor takes the thin strips' small tile
+      od.push_back(d);
       continue;
     }
     const int64_t Mf = d.M / bm * bm, Nf = d.N / bn * bn;
@@ -475,16 +476,19 @@ bool carve_full(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods, std::vec
     if (d.N > Nf) st.push_back(gsub(d, 0, Nf, d.M, d.N - Nf));          // right strip, all rows
     if (d.M > Mf && Nf > 0) st.push_back(gsub(d, Mf, 0, d.M - Mf, Nf));  // bottom strip
   }
-  if (core.empty() || st.empty()) return false;
-  double t_strips = 0;
-  const int cs = best_tile(P.dtype, P.mode, st, &t_strips);
+  if (core.empty() || (st.empty() && od.empty())) return false;
+  double t_strips = 0, t_odd = 0;
+  const int cs = st.empty() ? cfg : best_tile(P.dtype, P.mode, st, &t_strips);
+  const int co = od.empty() ? cfg : best_tile(P.dtype, P.mode, od, &t_odd);
   const double t_core = tile_time(*c, count_tiles(c->cfg_full, core), true);
   const double t_all = tile_time(*c, count_tiles(cfg, prods), false);
-  if (t_core + t_strips >= 0.97 * t_all) return false;
+  if (t_core + t_strips + t_odd >= 0.97 * t_all) return false;
   prods = std::move(core);
   strips = std::move(st);
+  odd = std::move(od);
   *cfg_core = c->cfg_full;
   *cfg_strips = cs;
+  *cfg_odd = co;
   return true;
 }
 }  // namespace
@@ -916,10 +920,10 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
           cfg != FM_TILE_128x128_W8)
         return set_err(FM_ERR_INVALID_ARG, "tile config " + std::to_string(cfg) +
                                                " has no split_bf16 kernel");
-      std::vector<FmGemmDesc> rest, strips;
-      int cfg_rest = cfg, cfg_head = cfg, cfg_main = cfg, cfg_strips = cfg;
+      std::vector<FmGemmDesc> rest, strips, odd;
+      int cfg_rest = cfg, cfg_head = cfg, cfg_main = cfg, cfg_strips = cfg, cfg_odd = cfg;
       if (persistent_grid == 0 && P.split_rem &&
-          carve_full(P, cfg, leaves, strips, &cfg_main, &cfg_strips))
+          carve_full(P, cfg, leaves, strips, odd, &cfg_main, &cfg_strips, &cfg_odd))
         cfg_rest = cfg_head = cfg_main;
       if (persistent_grid == 0 && P.split_rem)
         split_remainder(P, cfg_main, leaves, rest, &cfg_rest, &cfg_head);
@@ -937,6 +941,7 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
       P.launches.back().sp.grid = persistent_grid;
       if (!rest.empty()) P.launches.push_back(leaf(std::move(rest), cfg_rest));
       if (!strips.empty()) P.launches.push_back(leaf(std::move(strips), cfg_strips));
+      if (!odd.empty()) P.launches.push_back(leaf(std::move(odd), cfg_odd));
     }
   }
   int maxd = 0;

@@ -10,6 +10,8 @@ Measures, with the library's own kernels:
 * the S/T/W stage bandwidth: algorithmic stage bytes of one Strassen level
   at 4096^3 over the stages' device time (the library's per-launch events);
 * the cost of one launch: a batch of one 64^3 product;
+* the time per unit of K of one 64 x 64 tile (one 64 x 4096 x 64 product):
+  a peeling launch lasts at least its longest tile, however few its flops;
 * the cost of one recursion level of a batched problem beyond its bytes and
   launches: 48 uneven problems at one level of Strassen and of <4,2,3;20>,
   measured time minus the model's estimate without that term (less the
@@ -103,7 +105,15 @@ def launch_cost():
     return _time(b.run, 50) * 1e6
 
 
-def write_table(path, rates, gbps, launch_us, level_us, note):
+def tile_k_cost(dtype, launch_us):
+    """Time per unit of K of one 64 x 64 tile (a launch of one tile, K = 4096),
+    less the launch: the floor under a peeling launch's duration."""
+    k = 4096
+    b = fastmm.Batch(_batch([(64, k, 64)], dtype, "plain"), graph=True)
+    return max(0.0, (_time(b.run, 20) * 1e6 - launch_us) / k)
+
+
+def write_table(path, rates, gbps, launch_us, level_us, tile_k, note):
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     with open(path, "w") as f:
         f.write("# fastmm autotuner cost model, MI355X (gfx950); %s\n" % note)
@@ -115,9 +125,11 @@ def write_table(path, rates, gbps, launch_us, level_us, note):
         f.write("launch_us %.2f\n" % launch_us)
         for dt in ("f64", "f32"):
             f.write("level_us %s %.2f\n" % (dt, level_us[dt]))
+        for dt in ("f64", "f32"):
+            f.write("tile_k_us %s %.4f\n" % (dt, tile_k[dt]))
 
 
-def level_cost(path, rates, gbps, launch_us):
+def level_cost(path, rates, gbps, launch_us, tile_k):
     """Per-problem cost of one batched recursion level beyond bytes and
     launches: measured minus estimated (with that term zero), over batches of
     uneven problems (every dimension a multiple of 64 in [512, 4096]) run
@@ -130,7 +142,7 @@ def level_cost(path, rates, gbps, launch_us):
     level = {}
     with tempfile.TemporaryDirectory() as d:
         tmp = os.path.join(d, "t.txt")
-        write_table(tmp, rates, gbps, launch_us, {"f64": 0.0, "f32": 0.0}, "level term zero")
+        write_table(tmp, rates, gbps, launch_us, {"f64": 0.0, "f32": 0.0}, tile_k, "level term zero")
         fastmm.autotune_load(tmp)
         for dt, tdt in (("f32", torch.float32), ("f64", torch.float64)):
             extra = []
@@ -160,12 +172,13 @@ def main(argv=None):
     rates = leaf_rates()
     gbps = {"f64": stage_rate(torch.float64), "f32": stage_rate(torch.float32)}
     launch_us = launch_cost()
-    level_us = level_cost(a.out, rates, gbps, launch_us)
+    tile_k = {"f64": tile_k_cost(torch.float64, launch_us), "f32": tile_k_cost(torch.float32, launch_us)}
+    level_us = level_cost(a.out, rates, gbps, launch_us, tile_k)
     note = "python -m fastmm.calibrate, %s" % time.strftime("%Y-%m-%d")
-    write_table(a.out, rates, gbps, launch_us, level_us, note)
+    write_table(a.out, rates, gbps, launch_us, level_us, tile_k, note)
     fastmm.autotune_load(a.out)
     print(json.dumps({"table": a.out, "stage_gbps": gbps, "launch_us": launch_us,
-                      "level_us": level_us,
+                      "level_us": level_us, "tile_k_us": tile_k,
                       "leaf": {"%s/%s/%d" % (dt, m, s): round(tf, 2) for dt, m, s, tf in rates}}))
 
 

@@ -55,7 +55,10 @@ typedef enum { FM_F64 = 0, FM_F32 = 1 } fm_dtype;
 /* Leaf arithmetic.  PLAIN: leaves in the data type on its MFMA path.
  * SPLIT_BF16: FP32 data, leaves from split-bfloat16 products with FP32
  * accumulation.  OZAKI_INT8: FP64 data, leaves from int8 slices with exact
- * int32 accumulation and FP64 reconstruction. */
+ * int32 accumulation and FP64 reconstruction; the int32 sums stay exact for
+ * leaf K (rounded up to 64) up to 2^31 / (S 127^2), i.e. 16640 with the
+ * default S = 8 digits: deeper leaves are refused with FM_ERR_SHAPE (use
+ * more recursion levels), and the batch autotuner never picks them. */
 typedef enum { FM_MODE_PLAIN = 0, FM_MODE_SPLIT_BF16 = 1, FM_MODE_OZAKI_INT8 = 2 } fm_mode;
 
 /* <alg>::FastMatmul (algorithms/strassen.hpp:662-722).  `alg` is the

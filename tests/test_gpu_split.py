@@ -160,3 +160,40 @@ def test_split_planes_shared_operand(fm, _tile_reset):
     fm.set_gemm_tile(-1)
     got = fm.fastmatmul("classical222_8_24", Ad, Bd, None, 1, mode="split_bf16").cpu().numpy()
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("tile", [-1, 0])
+def test_split_bf16_huge_and_nonfinite_entries(fm, tile):
+    """FP32 entries whose rounding to bf16 would overflow (up to FLT_MAX)
+    are split with a truncated leading slice, so the split-bf16 product stays
+    finite and FP32-accurate where the plain FP32 product is; Inf and NaN
+    entries give non-finite results exactly where the plain product has them
+    (slice planes: tile -1; in-loop splitting: tile 0)."""
+    rng = np.random.default_rng(91)
+    m, k, n = 96, 128, 80
+    A = (rng.random((m, k)) * 2 - 1).astype(np.float32)
+    B = ((rng.random((k, n)) * 2 - 1) * 1e-3).astype(np.float32)
+    A[3, 5] = np.float32(3.4028235e38)
+    A[7, 9] = -np.float32(3.395e38)
+    A[11, 0] = np.float32(3.39e38)
+    Ad, Bd = fm.col_major(torch.from_numpy(A).to(DEV)), fm.col_major(torch.from_numpy(B).to(DEV))
+
+    def run(mode, A_, B_):
+        P = fm.Plan(None, m, k, n, 0, dtype=torch.float32, mode=mode,
+                    opts=dict(gemm_tile=tile) if mode == "split_bf16" else None)
+        return P.run(A_, B_, fm.empty(m, n, torch.float32, DEV)).cpu().numpy()
+
+    got, plain = run("split_bf16", Ad, Bd), run("plain", Ad, Bd)
+    want = A.astype(np.float64) @ B.astype(np.float64)
+    assert np.all(np.isfinite(got)) and np.all(np.isfinite(plain))
+    scale = np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)
+    assert np.all(np.abs(got - want) <= 4 * k * 2.0 ** -24 * scale)
+    # Inf and NaN entries
+    A2 = A.copy()
+    A2[20, 4] = np.inf
+    A2[30, 6] = -np.inf
+    A2[40, 8] = np.nan
+    Ad2 = fm.col_major(torch.from_numpy(A2).to(DEV))
+    got2, plain2 = run("split_bf16", Ad2, Bd), run("plain", Ad2, Bd)
+    assert np.array_equal(np.isfinite(got2), np.isfinite(plain2))
+    assert np.isnan(got2[40]).all()
