@@ -46,6 +46,8 @@ struct FmMfma<float> {
 // Map a work-group id to a tile id so that each of the 8 XCDs (which receive
 // work groups round-robin) walks a contiguous range of tiles: neighbouring
 // tiles share A/B panels in that XCD's L2.
+constexpr int FM_TILE_GROUP = 8;
+
 __device__ __forceinline__ int fm_xcd_remap(int wg, int nwg) {
   const int full = nwg & ~7;
   if (wg < full) return (wg & 7) * (full >> 3) + (wg >> 3);
@@ -112,7 +114,16 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   }
   const FmGemmDesc d = args.descs[g];
   const int local = wg - d.tile_begin;
-  const int tmi = local % d.tiles_m, tni = local / d.tiles_m;
+  // Tiles go in groups of FM_TILE_GROUP tile rows, all tile columns of a
+  // group, rows fastest: the work groups that run together on one XCD share
+  // a few A row panels and walk B's column panels once per group, so a tall
+  // product reads A from HBM about once (row-major order would read it once
+  // per tile column; column-major, once per tile column too when the
+  // product is taller than the tiles in flight).
+  const int per_group = FM_TILE_GROUP * d.tiles_n, grp = local / per_group;
+  const int first = grp * FM_TILE_GROUP, rows_g = min(FM_TILE_GROUP, d.tiles_m - first);
+  const int rem = local - grp * per_group;
+  const int tmi = first + rem % rows_g, tni = rem / rows_g;
   const int m0 = tmi * BM, n0 = tni * BN;
   const int M = d.M, N = d.N, K = d.K;
   const T* __restrict__ A = reinterpret_cast<const T*>(fm_base(args.bases, d.a.role)) + d.a.off;
