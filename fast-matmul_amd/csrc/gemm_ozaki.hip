@@ -24,6 +24,7 @@
 #include <stdint.h>
 
 #include "gemm_ozaki.hpp"
+#include "gmem.hpp"
 
 typedef int fm_i32x4 __attribute__((ext_vector_type(4)));
 
@@ -70,11 +71,11 @@ __global__ void __launch_bounds__(256) fm_ozaki_prep_rows(FmOzakiArgs args) {
   const FmOzakiDesc d = args.descs[blockIdx.y];
   const int r0 = blockIdx.x * 64;
   if (r0 >= d.M) return;
-  const double* A = reinterpret_cast<const double*>(fm_base(args.bases, d.a.role)) + d.a.off;
+  const fm_gptr<const double> A = fm_global(reinterpret_cast<const double*>(fm_base(args.bases, d.a.role)) + d.a.off);
   const int64_t lda = d.a.ld;
-  char* ws = reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]);
-  int32_t* ex = reinterpret_cast<int32_t*>(ws + d.a_exp);
-  int8_t* sl = reinterpret_cast<int8_t*>(ws + d.a_slc);
+  const fm_gptr<char> ws = fm_global(reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]));
+  const fm_gptr<int32_t> ex = (fm_gptr<int32_t>)(ws + d.a_exp);
+  const fm_gptr<int8_t> sl = (fm_gptr<int8_t>)(ws + d.a_slc);
   const int tid = threadIdx.x, r = tid & 63, kq = tid >> 6;  // 4 threads per row
   const int row = r0 + r;
   const bool rok = row < d.M;
@@ -105,8 +106,8 @@ __global__ void __launch_bounds__(256) fm_ozaki_prep_rows(FmOzakiArgs args) {
       const int k = k0 + 16 * kq;
 #pragma unroll
       for (int s = 0; s < S; ++s) {
-        int8_t* p = sl + fm_oz_chunk<S>(k, s, d.rows_pad, row);
-        *reinterpret_cast<int4*>(p) = *reinterpret_cast<const int4*>(&dig[s][0]);
+        const fm_gptr<int8_t> p = sl + fm_oz_chunk<S>(k, s, d.rows_pad, row);
+        *(fm_gptr<fm_i32x4>)(p) = *reinterpret_cast<const fm_i32x4*>(&dig[s][0]);
       }
     }
   }
@@ -124,18 +125,18 @@ __global__ void __launch_bounds__(256) fm_ozaki_prep_cols(FmOzakiArgs args) {
   const FmOzakiDesc d = args.descs[blockIdx.y];
   const int c0 = blockIdx.x * 64;
   if (c0 >= d.cols_pad) return;
-  const double* B = reinterpret_cast<const double*>(fm_base(args.bases, d.b.role)) + d.b.off;
+  const fm_gptr<const double> B = fm_global(reinterpret_cast<const double*>(fm_base(args.bases, d.b.role)) + d.b.off);
   const int64_t ldb = d.b.ld;
-  char* ws = reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]);
-  int32_t* ex = reinterpret_cast<int32_t*>(ws + d.b_exp);
-  int8_t* sl = reinterpret_cast<int8_t*>(ws + d.b_slc);
+  const fm_gptr<char> ws = fm_global(reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]));
+  const fm_gptr<int32_t> ex = (fm_gptr<int32_t>)(ws + d.b_exp);
+  const fm_gptr<int8_t> sl = (fm_gptr<int8_t>)(ws + d.b_slc);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   __shared__ int es[64];
   for (int q = 0; q < 16; ++q) {
     const int cl = wave * 16 + q, col = c0 + cl;
     double mx = 0.0;
     if (col < d.N) {
-      const double* bc = B + (int64_t)col * ldb;
+      const fm_gptr<const double> bc = B + (int64_t)col * ldb;
 #pragma unroll 8
       for (int k = lane; k < d.K; k += 64) mx = fmax(mx, fabs(bc[k]));
     }
@@ -170,8 +171,8 @@ __global__ void __launch_bounds__(256) fm_ozaki_prep_cols(FmOzakiArgs args) {
     }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      int8_t* p = sl + fm_oz_chunk<S>(kb, s, d.cols_pad, col);
-      *reinterpret_cast<int4*>(p) = *reinterpret_cast<const int4*>(&dig[s][0]);
+      const fm_gptr<int8_t> p = sl + fm_oz_chunk<S>(kb, s, d.cols_pad, col);
+      *(fm_gptr<fm_i32x4>)(p) = *reinterpret_cast<const fm_i32x4*>(&dig[s][0]);
     }
   }
 }
@@ -202,9 +203,9 @@ __global__ void __launch_bounds__(256) fm_ozaki_gemm(FmOzakiArgs args) {
   const FmOzakiDesc d = args.descs[lo];
   const int local = wg - d.tile_begin;
   const int m0 = (local % d.tiles_m) * BM, n0 = (local / d.tiles_m) * BN;
-  char* ws = reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]);
-  const int8_t* sa = reinterpret_cast<const int8_t*>(ws + d.a_slc);
-  const int8_t* sb = reinterpret_cast<const int8_t*>(ws + d.b_slc);
+  const fm_gptr<char> ws = fm_global(reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]));
+  const fm_gptr<const int8_t> sa = (fm_gptr<const int8_t>)(ws + d.a_slc);
+  const fm_gptr<const int8_t> sb = (fm_gptr<const int8_t>)(ws + d.b_slc);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
 
@@ -221,10 +222,10 @@ __global__ void __launch_bounds__(256) fm_ozaki_gemm(FmOzakiArgs args) {
   for (int k0 = 0; k0 < d.Kp; k0 += BK) {
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      const int4 va = *reinterpret_cast<const int4*>(sa + fm_oz_chunk<S>(k0 + lc, s, d.rows_pad, m0 + lr));
-      const int4 vb = *reinterpret_cast<const int4*>(sb + fm_oz_chunk<S>(k0 + lc, s, d.cols_pad, n0 + lr));
-      *reinterpret_cast<int4*>(&As[s][lr][lc]) = va;
-      *reinterpret_cast<int4*>(&Bs[s][lr][lc]) = vb;
+      const fm_i32x4 va = *(fm_gptr<const fm_i32x4>)(sa + fm_oz_chunk<S>(k0 + lc, s, d.rows_pad, m0 + lr));
+      const fm_i32x4 vb = *(fm_gptr<const fm_i32x4>)(sb + fm_oz_chunk<S>(k0 + lc, s, d.cols_pad, n0 + lr));
+      *reinterpret_cast<fm_i32x4*>(&As[s][lr][lc]) = va;
+      *reinterpret_cast<fm_i32x4*>(&Bs[s][lr][lc]) = vb;
     }
     __syncthreads();
     const int kofs = 16 * (lane >> 4);
@@ -251,9 +252,9 @@ __global__ void __launch_bounds__(256) fm_ozaki_gemm(FmOzakiArgs args) {
     __syncthreads();
   }
 
-  const int32_t* ea = reinterpret_cast<const int32_t*>(ws + d.a_exp);
-  const int32_t* eb = reinterpret_cast<const int32_t*>(ws + d.b_exp);
-  double* C = reinterpret_cast<double*>(fm_base(args.bases, d.c.role)) + d.c.off;
+  const fm_gptr<const int32_t> ea = (fm_gptr<const int32_t>)(ws + d.a_exp);
+  const fm_gptr<const int32_t> eb = (fm_gptr<const int32_t>)(ws + d.b_exp);
+  const fm_gptr<double> C = fm_global(reinterpret_cast<double*>(fm_base(args.bases, d.c.role)) + d.c.off);
   const int64_t ldc = d.c.ld;
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
@@ -273,7 +274,7 @@ __global__ void __launch_bounds__(256) fm_ozaki_gemm(FmOzakiArgs args) {
         for (int u = S - 1; u >= 0; --u)
           sum = __dadd_rn(sum, ldexp((double)acc[u][i][j][v], -7 * (u + 2)));
         double r = __dmul_rn(ldexp(sum, er + eb[col]), d.alpha);
-        double* p = C + row + (int64_t)col * ldc;
+        const fm_gptr<double> p = C + row + (int64_t)col * ldc;
         if (d.beta != 0.0) r = __dadd_rn(r, __dmul_rn(d.beta, *p));
         *p = r;
       }
@@ -328,14 +329,14 @@ __global__ void __launch_bounds__(256, 2) fm_ozaki_gemm32(FmOzakiArgs args) {
   const int first = grp * GROUP, rows_g = min(GROUP, d.tiles_m - first);
   const int rem = local - grp * per_group;
   const int m0 = (first + rem % rows_g) * BM, n0 = (rem / rows_g) * 64;
-  char* ws = reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]);
-  const fm_i32x4* sa = reinterpret_cast<const fm_i32x4*>(ws + d.a_slc) + m0;
+  const fm_gptr<char> ws = fm_global(reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]));
+  const fm_gptr<const fm_i32x4> sa = (fm_gptr<const fm_i32x4>)(ws + d.a_slc) + m0;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int fr = lane & 31, fh = lane >> 5;
   const int64_t pa = d.rows_pad, pb = d.cols_pad;
   // this lane's B column (n0 + wn 32 + fr < cols_pad) and K half
-  const fm_i32x4* sbl = reinterpret_cast<const fm_i32x4*>(ws + d.b_slc) + n0 + wn * 32 + fr + fh * pb;
+  const fm_gptr<const fm_i32x4> sbl = (fm_gptr<const fm_i32x4>)(ws + d.b_slc) + n0 + wn * 32 + fr + fh * pb;
 
   fm_i32x16 acc[S];
 #pragma unroll
@@ -404,9 +405,9 @@ __global__ void __launch_bounds__(256, 2) fm_ozaki_gemm32(FmOzakiArgs args) {
     __syncthreads();
   }
 
-  const int32_t* ea = reinterpret_cast<const int32_t*>(ws + d.a_exp);
-  const int32_t* eb = reinterpret_cast<const int32_t*>(ws + d.b_exp);
-  double* C = reinterpret_cast<double*>(fm_base(args.bases, d.c.role)) + d.c.off;
+  const fm_gptr<const int32_t> ea = (fm_gptr<const int32_t>)(ws + d.a_exp);
+  const fm_gptr<const int32_t> eb = (fm_gptr<const int32_t>)(ws + d.b_exp);
+  const fm_gptr<double> C = fm_global(reinterpret_cast<double*>(fm_base(args.bases, d.c.role)) + d.c.off);
   const int64_t ldc = d.c.ld;
   const int row = m0 + wm * 32 + fr;
   if (row >= d.M) return;
@@ -419,7 +420,7 @@ __global__ void __launch_bounds__(256, 2) fm_ozaki_gemm32(FmOzakiArgs args) {
 #pragma unroll
     for (int u = S - 1; u >= 0; --u) sum = __dadd_rn(sum, ldexp((double)acc[u][e], -7 * (u + 2)));
     double r = __dmul_rn(ldexp(sum, er + eb[col]), d.alpha);
-    double* p = C + row + (int64_t)col * ldc;
+    const fm_gptr<double> p = C + row + (int64_t)col * ldc;
     if (d.beta != 0.0) r = __dadd_rn(r, __dmul_rn(d.beta, *p));
     *p = r;
   }

@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include "gemm_mfma.hpp"
+#include "gmem.hpp"
 
 typedef __bf16 fm_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float fm_f32x4 __attribute__((ext_vector_type(4)));
@@ -107,9 +108,11 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_bf16x3_kernel(FmGemmArgs
   const int local = wg - d.tile_begin;
   const int m0 = (local % d.tiles_m) * BM, n0 = (local / d.tiles_m) * BN;
   const int M = d.M, N = d.N, K = d.K;
-  const float* __restrict__ A = reinterpret_cast<const float*>(fm_base(args.bases, d.a.role)) + d.a.off;
-  const float* __restrict__ B = reinterpret_cast<const float*>(fm_base(args.bases, d.b.role)) + d.b.off;
-  float* __restrict__ C = reinterpret_cast<float*>(fm_base(args.bases, d.c.role)) + d.c.off;
+  const fm_gptr<const float> __restrict__ A =
+      fm_global(reinterpret_cast<const float*>(fm_base(args.bases, d.a.role)) + d.a.off);
+  const fm_gptr<const float> __restrict__ B =
+      fm_global(reinterpret_cast<const float*>(fm_base(args.bases, d.b.role)) + d.b.off);
+  const fm_gptr<float> __restrict__ C = fm_global(reinterpret_cast<float*>(fm_base(args.bases, d.c.role)) + d.c.off);
   const int64_t lda = d.a.ld, ldb = d.b.ld, ldc = d.c.ld;
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -122,7 +125,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_bf16x3_kernel(FmGemmArgs
       const int o = tid + u * NT;
       const int row = o % BM, kq = o / BM;
       const bool rok = m0 + row < M;
-      const float* p = A + (m0 + row) + (int64_t)(k0 + 8 * kq) * lda;
+      const fm_gptr<const float> p = A + (m0 + row) + (int64_t)(k0 + 8 * kq) * lda;
 #pragma unroll
       for (int e = 0; e < 8; ++e)
         ra[u][e] = (rok && k0 + 8 * kq + e < K) ? p[(int64_t)e * lda] : 0.f;
@@ -132,7 +135,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_bf16x3_kernel(FmGemmArgs
       const int o = tid + u * NT;
       const int kq = o % (BK / 8), col = o / (BK / 8);
       const bool cok = n0 + col < N;
-      const float* p = B + (k0 + 8 * kq) + (int64_t)(n0 + col) * ldb;
+      const fm_gptr<const float> p = B + (k0 + 8 * kq) + (int64_t)(n0 + col) * ldb;
 #pragma unroll
       for (int e = 0; e < 8; ++e) rb[u][e] = (cok && k0 + 8 * kq + e < K) ? p[e] : 0.f;
     }
@@ -210,7 +213,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_gemm_bf16x3_kernel(FmGemmArgs
       for (int v = 0; v < 4; ++v) {
         const int col = n0 + wn * WTN + j * 16 + 4 * (lane >> 4) + v;
         if (col < N) {
-          float* p = C + row + (int64_t)col * ldc;
+          const fm_gptr<float> p = C + row + (int64_t)col * ldc;
           float r = alpha * acc[i][j][v];
           if (beta != 0.f) r += beta * *p;
           *p = r;
@@ -266,9 +269,9 @@ __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first
   const int r0 = blockIdx.x * 64, kc0 = blockIdx.y * FM_PREP_KCH;
   const int nkc = d.Kp >> 5;
   if (r0 >= d.rows_pad || kc0 >= nkc) return;
-  const float* __restrict__ src =
-      reinterpret_cast<const float*>(fm_base(args.bases, d.src.role)) + d.src.off;
-  char* ws = reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]);
+  const fm_gptr<const float> __restrict__ src =
+      fm_global(reinterpret_cast<const float*>(fm_base(args.bases, d.src.role)) + d.src.off);
+  const fm_gptr<char> ws = fm_global(reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]));
   const int tid = threadIdx.x;
   const int r = tid >> 2, q = tid & 3;
   const int row = r0 + r;
@@ -279,19 +282,19 @@ __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first
     const int k = kc * 32 + 8 * q;
     float x[8];
     if (d.kcontig) {
-      const float* p = src + (int64_t)row * ld + k;
+      const fm_gptr<const float> p = src + (int64_t)row * ld + k;
 #pragma unroll
       for (int e = 0; e < 8; ++e) x[e] = (rok && k + e < d.K) ? p[e] : 0.f;
     } else {
-      const float* p = src + row + (int64_t)k * ld;
+      const fm_gptr<const float> p = src + row + (int64_t)k * ld;
 #pragma unroll
       for (int e = 0; e < 8; ++e) x[e] = (rok && k + e < d.K) ? p[(int64_t)e * ld] : 0.f;
     }
     uint4 sl[3];
     fm_split3(x, sl);
-    char* dst = ws + d.dst + ((int64_t)kc * d.rows_pad + row) * 192 + q * 16;
+    const fm_gptr<char> dst = ws + d.dst + ((int64_t)kc * d.rows_pad + row) * 192 + q * 16;
 #pragma unroll
-    for (int p = 0; p < 3; ++p) *reinterpret_cast<uint4*>(dst + p * 64) = sl[p];
+    for (int p = 0; p < 3; ++p) *(fm_gptr<fm_u32x4>)(dst + p * 64) = *reinterpret_cast<const fm_u32x4*>(&sl[p]);
   }
 }
 
@@ -356,12 +359,12 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
   const int rem = local - grp * per_group;
   const int m0 = (first + rem % rows_g) * BM, n0 = (rem / rows_g) * BN;
   const int nkc = d.Kp >> 5;
-  const char* ws = reinterpret_cast<const char*>(args.bases.p[FM_ROLE_WS]);
+  const fm_gptr<const char> ws = fm_global(reinterpret_cast<const char*>(args.bases.p[FM_ROLE_WS]));
   // chunk o of a slice: plane o / (4 * rows), row (o / 4) % rows, chunk o % 4;
   // consecutive K slices of a plane set are apa (bpa) rows apart
   const int64_t apa = (int64_t)d.tiles_m * BM, bpa = (int64_t)d.tiles_n * BN;
-  const fm_u32x4* Ap = reinterpret_cast<const fm_u32x4*>(ws + d.a_pl) + (int64_t)m0 * 12;
-  const fm_u32x4* Bp = reinterpret_cast<const fm_u32x4*>(ws + d.b_pl) + (int64_t)n0 * 12;
+  const fm_gptr<const fm_u32x4> Ap = (fm_gptr<const fm_u32x4>)(ws + d.a_pl) + (int64_t)m0 * 12;
+  const fm_gptr<const fm_u32x4> Bp = (fm_gptr<const fm_u32x4>)(ws + d.b_pl) + (int64_t)n0 * 12;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
 
@@ -462,7 +465,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
   }
 
   const float alpha = (float)d.alpha, beta = (float)d.beta;
-  float* __restrict__ C = reinterpret_cast<float*>(fm_base(args.bases, d.c.role)) + d.c.off;
+  const fm_gptr<float> __restrict__ C = fm_global(reinterpret_cast<float*>(fm_base(args.bases, d.c.role)) + d.c.off);
   const int64_t ldc = d.c.ld;
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
@@ -474,7 +477,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm(FmSplitArgs args) 
       for (int v = 0; v < 4; ++v) {
         const int col = n0 + wn * WTN + j * 16 + 4 * fc + v;
         if (col < d.N) {
-          float* p = C + row + (int64_t)col * ldc;
+          const fm_gptr<float> p = C + row + (int64_t)col * ldc;
           float r = alpha * acc[i][j][v];
           if (beta != 0.f) r += beta * *p;
           *p = r;
@@ -531,16 +534,16 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm_bd(FmSplitArgs arg
   const int rem = local - grp * per_group;
   const int m0 = (first + rem % rows_g) * BM, n0 = (rem / rows_g) * BN;
   const int nkc = d.Kp >> 5;
-  const char* ws = reinterpret_cast<const char*>(args.bases.p[FM_ROLE_WS]);
+  const fm_gptr<const char> ws = fm_global(reinterpret_cast<const char*>(args.bases.p[FM_ROLE_WS]));
   const int64_t apa = (int64_t)d.tiles_m * BM, bpa = (int64_t)d.tiles_n * BN;
-  const fm_u32x4* Ap = reinterpret_cast<const fm_u32x4*>(ws + d.a_pl) + (int64_t)m0 * 12;
+  const fm_gptr<const fm_u32x4> Ap = (fm_gptr<const fm_u32x4>)(ws + d.a_pl) + (int64_t)m0 * 12;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   const int fr = lane & 15, fc = lane >> 4;
   const int fsw = fc ^ fm_sp_swz(fr);
   // this lane's B fragment bytes: column n0 + wn WTN + 16 j + fr, chunk fc
   // of each plane (bf16 k = 8 fc .. 8 fc + 7 of the slice)
-  const char* bl = ws + d.b_pl + (int64_t)(n0 + wn * WTN + fr) * 192 + fc * 16;
+  const fm_gptr<const char> bl = ws + d.b_pl + (int64_t)(n0 + wn * WTN + fr) * 192 + fc * 16;
 
   fm_u32x4 ra[A_CH];
   auto gload = [&](int ks) {  // A of 32-deep slice ks
@@ -561,7 +564,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm_bd(FmSplitArgs arg
   auto bload = [&](int kt, int j) {
 #pragma unroll
     for (int p = 0; p < 3; ++p)
-      bf[p][j] = *reinterpret_cast<const fm_bf16x8*>(bl + ((int64_t)kt * bpa + 16 * j) * 192 + p * 64);
+      bf[p][j] = *(fm_gptr<const fm_bf16x8>)(bl + ((int64_t)kt * bpa + 16 * j) * 192 + p * 64);
   };
 
   fm_f32x4 acc[TM][TN];
@@ -624,7 +627,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm_bd(FmSplitArgs arg
   }
 
   const float alpha = (float)d.alpha, beta = (float)d.beta;
-  float* __restrict__ C = reinterpret_cast<float*>(fm_base(args.bases, d.c.role)) + d.c.off;
+  const fm_gptr<float> __restrict__ C = fm_global(reinterpret_cast<float*>(fm_base(args.bases, d.c.role)) + d.c.off);
   const int64_t ldc = d.c.ld;
 #pragma unroll
   for (int i = 0; i < TM; ++i) {
@@ -636,7 +639,7 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm_bd(FmSplitArgs arg
       for (int v = 0; v < 4; ++v) {
         const int col = n0 + wn * WTN + j * 16 + 4 * fc + v;
         if (col < d.N) {
-          float* p = C + row + (int64_t)col * ldc;
+          const fm_gptr<float> p = C + row + (int64_t)col * ldc;
           float r = alpha * acc[i][j][v];
           if (beta != 0.f) r += beta * *p;
           *p = r;
