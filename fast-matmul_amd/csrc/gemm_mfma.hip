@@ -62,7 +62,10 @@ __device__ __forceinline__ int fm_xcd_remap(int wg, int nwg) {
 // registers).
 // PIPE: software-pipelined main loop (see the loop below); PIPE == 2 also
 // interleaves the next step's LDS reads with the current step's MFMAs
-// (one read, then two MFMAs) through scheduling-group barriers.  FULL: every
+// (one read, then two MFMAs) through scheduling-group barriers; PIPE == 3
+// is PIPE == 2 with the next slice's LDS stores and the slice after's
+// global loads spread over the K steps of the current slice (full-tile
+// launches; others run as PIPE == 2).  FULL: every
 // product is a whole number of tiles and K slices with 16-byte aligned
 // columns (the launch checks), so only the unpredicated 16-byte load path
 // is compiled.
@@ -95,7 +98,11 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   // MI355X at 8192^3: FP32 tiles gain 1-4% with global loads; the FP64
   // 128 x 128 two-groups-per-CU tile (the large-product choice) loses 4%
   // (70.8 -> 67.7 TFLOP/s), other FP64 tiles move within +-2%.
-  constexpr bool GM = sizeof(T) == 4;
+  constexpr bool SPREAD = PIPE == 3 && FULL;
+  constexpr int PIPE_EFF = PIPE == 3 ? 2 : PIPE;
+  // (the spread loop waits on each part's global load separately: it needs
+  // the global address space whatever the element type)
+  constexpr bool GM = sizeof(T) == 4 || SPREAD;
 
   __shared__ __align__(16) T As[2][BK][BM];
   __shared__ __align__(16) T Bs[2][BN][BKP];
@@ -251,7 +258,10 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   };
 
   const int nk = (K + BK - 1) / BK;
-  bool vec_stage = interior && BK <= K;
+  // (FULL: K is a positive multiple of BK; the constant keeps a tile's
+  // epilogue stores out of the next tile's main loop, where the compiler
+  // would otherwise have to wait for every load and store in flight)
+  bool vec_stage = FULL || (interior && BK <= K);
   if (vec_stage) {
     gload_vec(0);
     sstore_vec(0);
@@ -260,7 +270,98 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
     sstore(0);
   }
   __syncthreads();
-  if constexpr (PIPE) {
+  if constexpr (SPREAD) {
+    // The PIPE == 2 loop, with the LDS stores of slice kt + 1 and the global
+    // loads of slice kt + 2 cut in NS4 parts, one per K step of slice kt
+    // (part q: the 16-byte vectors i with i % NS4 == q).  Buffer cur ^ 1 is
+    // free during slice kt: its last fragment reads came before the
+    // previous barrier.  Each part's load has NS4 steps to land before its
+    // store, and the barrier then waits for one part's stores only.
+    constexpr int NS4 = BK / 4;
+    T af[2][TM], bf[2][TN];
+    auto frag = [&](int buf, int k4, int set) {
+      const int kidx = k4 * 4 + (lane >> 4);
+#pragma unroll
+      for (int i = 0; i < TM; ++i) af[set][i] = As[buf][kidx][wm * WTM + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bf[set][j] = Bs[buf][wn * WTN + j * 16 + (lane & 15)][kidx];
+    };
+    auto mma_step = [&](int set) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = Op::mma(bf[set][j], af[set][i], acc[i][j]);
+    };
+    auto store_part = [&](int buf, int q) {
+#pragma unroll
+      for (int i = 0; i < AV_PER; ++i)
+        if (i % NS4 == q) {
+          vec_t v;
+#pragma unroll
+          for (int e = 0; e < VL; ++e) v[e] = ra[i * VL + e];
+          *reinterpret_cast<vec_t*>(&As[buf][av_k + i * AV_KSTEP][av_row]) = v;
+        }
+#pragma unroll
+      for (int i = 0; i < BV_PER; ++i)
+        if (i % NS4 == q) {
+#pragma unroll
+          for (int e = 0; e < VL; ++e) Bs[buf][bv_col + i * BV_NSTEP][bv_k + e] = rb[i * VL + e];
+        }
+    };
+    auto load_part = [&](int k0, int q) {
+#pragma unroll
+      for (int i = 0; i < AV_PER; ++i)
+        if (i % NS4 == q) {
+          const vec_t v = *(const fm_ptr<GM, const vec_t>)(av_ptr + (int64_t)(k0 + i * AV_KSTEP) * lda);
+#pragma unroll
+          for (int e = 0; e < VL; ++e) ra[i * VL + e] = v[e];
+        }
+#pragma unroll
+      for (int i = 0; i < BV_PER; ++i)
+        if (i % NS4 == q) {
+          const vec_t v = *(const fm_ptr<GM, const vec_t>)(bv_ptr + k0 + (int64_t)(i * BV_NSTEP) * ldb);
+#pragma unroll
+          for (int e = 0; e < VL; ++e) rb[i * VL + e] = v[e];
+        }
+    };
+    // The stores and loads run unconditionally (the last slices store into
+    // a buffer no one reads again and reload slice nk - 1, in bounds):
+    // under a branch the compiler would wait for every load in flight
+    // before each store instead of the one that store needs.
+    // (in the loop's part order: the compiler's wait counts at the loop
+    // head hold for both ways in only when the loads in flight are ordered
+    // alike)
+#pragma unroll
+    for (int q = 0; q < NS4; ++q) {
+      load_part(min(1, nk - 1) * BK, q);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    frag(0, 0, 0);
+    for (int kt = 0; kt < nk; ++kt) {
+      const int cur = kt & 1;
+      const int kl = min(kt + 2, nk - 1) * BK;
+#pragma unroll
+      for (int q = 0; q < NS4; ++q) {
+        store_part(cur ^ 1, q);
+        load_part(kl, q);
+        __builtin_amdgcn_sched_barrier(0);
+        if (q < NS4 - 1) {
+          frag(cur, q + 1, (q + 1) & 1);
+          mma_step(q & 1);
+#pragma unroll
+          for (int r = 0; r < TM + TN; ++r) {
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
+            __builtin_amdgcn_sched_group_barrier(0x008, (TM * TN) / (TM + TN), 0);  // MFMAs
+          }
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+      __syncthreads();
+      frag(cur ^ 1, 0, NS4 & 1);
+      __builtin_amdgcn_sched_barrier(0);
+      mma_step((NS4 - 1) & 1);
+    }
+  } else if constexpr (PIPE_EFF) {
     // Fragments of K step s + 1 are read from LDS before the MFMAs of step s
     // (two register sets), and the MFMAs of a slice's last step are issued
     // after the barrier that publishes the next slice, behind that slice's
@@ -294,7 +395,7 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
       for (int k4 = 0; k4 < NS4 - 1; ++k4) {
         frag(cur, k4 + 1, (k4 + 1) & 1);
         mma_step(k4 & 1);
-        if constexpr (PIPE == 2) {
+        if constexpr (PIPE_EFF == 2) {
 #pragma unroll
           for (int r = 0; r < TM + TN; ++r) {
             __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // one LDS read
@@ -424,6 +525,8 @@ static hipError_t fm_gemm_launch(int cfg, const FmGemmArgs& a, hipStream_t s) {
     case FM_TILE_PI_128x128_O2: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 4, 2>(a, s); break;
     case FM_TILE_PI_128x256_W8: fm_gemm_go<T, 128, 256, 16, 2, 4, 1, 4, 2>(a, s); break;
     case FM_TILE_PI_128x96_O2: fm_gemm_go<T, 128, 96, 16, 2, 2, 2, 4, 2>(a, s); break;
+    case FM_TILE_PS_128x128_O2: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 4, 3>(a, s); break;
+    case FM_TILE_PS_128x256_W8: fm_gemm_go<T, 128, 256, 16, 2, 4, 1, 4, 3>(a, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

@@ -71,7 +71,11 @@ enum FmTileCfg : int {
   FM_TILE_PI_128x96_O2 = 30,   // BK 16, 2x2 waves of 64 x 48, registers for 2 waves per SIMD
   // split-bf16 on slice planes, B fragments loaded straight into registers
   FM_TILE_SP_128x256_BD = 31,  // 2x4 waves, A through double-buffered LDS
-  FM_TILE_NUM = 32
+  // as 22 and 23, with the next slice's LDS stores and global loads spread
+  // over the K steps (full-tile launches)
+  FM_TILE_PS_128x128_O2 = 32,
+  FM_TILE_PS_128x256_W8 = 33,
+  FM_TILE_NUM = 34
 };
 
 // configurations that run on prepared slice planes
@@ -96,7 +100,8 @@ inline void fm_tile_dims(int cfg, int* bm, int* bn) {
                                         {128, 128}, {128, 256}, {256, 128}, {128, 256},
                                         {128, 128}, {128, 128}, {128, 128}, {128, 256},
                                         {128, 128}, {256, 128}, {128, 256}, {128, 128},
-                                        {256, 128}, {128, 256}, {128, 96},  {128, 256}};
+                                        {256, 128}, {128, 256}, {128, 96},  {128, 256},
+                                        {128, 128}, {128, 256}};
   *bm = d[cfg][0];
   *bn = d[cfg][1];
 }

@@ -336,23 +336,27 @@ struct TileCand {
 // product of 512 x 2048 x 512), best = 1.  Work groups per CU: the lesser of
 // what the register allocation (waves per SIMD x 4 / waves per group) and
 // the 160 KB of LDS allow.
+// (8192^3: FP64 72.6 TFLOP/s with the spread 128 x 128 tile, 69.0 with
+// the same tile loading each slice at once; FP32 141.9 with the spread
+// 128 x 256 tile, 136 with the 256 x 256 one)
 const TileCand kF64Tiles[] = {
-    {FM_TILE_PI_128x128_O2, FM_TILE_128x128_O2_U1, 1.00, 2},
-    {FM_TILE_128x256_W8, FM_TILE_128x256_W8_U1, 0.97, 1},
-    {FM_TILE_256x128_W8, FM_TILE_256x128_W8_U1, 0.96, 1},
-    {FM_TILE_128x128_W16, FM_TILE_128x128_W16, 0.90, 1},
-    {FM_TILE_128x64_W8, FM_TILE_128x64_W8, 0.88, 2},
-    {FM_TILE_PI_128x96_O2, FM_TILE_PI_128x96_O2, 0.88, 2},
-    {FM_TILE_64x64, FM_TILE_64x64, 0.78, 3}};
+    {FM_TILE_PS_128x128_O2, FM_TILE_128x128_O2_U1, 1.00, 2},
+    {FM_TILE_PS_128x256_W8, FM_TILE_128x256_W8_U1, 0.97, 1},
+    {FM_TILE_128x256_W8, FM_TILE_128x256_W8_U1, 0.93, 1},
+    {FM_TILE_256x128_W8, FM_TILE_256x128_W8_U1, 0.92, 1},
+    {FM_TILE_128x128_W16, FM_TILE_128x128_W16, 0.86, 1},
+    {FM_TILE_128x64_W8, FM_TILE_128x64_W8, 0.84, 2},
+    {FM_TILE_PI_128x96_O2, FM_TILE_PI_128x96_O2, 0.84, 2},
+    {FM_TILE_64x64, FM_TILE_64x64, 0.75, 3}};
 const TileCand kF32Tiles[] = {
-    {FM_TILE_256x256_W16, -1, 1.00, 1},
-    {FM_TILE_PI_128x128_O2, FM_TILE_128x128_W8, 0.97, 3},
-    {FM_TILE_PP_128x256_W8, FM_TILE_PP_128x256_W8, 0.94, 1},
-    {FM_TILE_256x128_W8, FM_TILE_256x128_W8, 0.93, 1},
-    {FM_TILE_128x128_W8, FM_TILE_128x128_W8, 0.79, 2},
-    {FM_TILE_64x128, FM_TILE_64x128, 0.74, 3},
-    {FM_TILE_128x64, FM_TILE_128x64, 0.71, 3},
-    {FM_TILE_64x64, FM_TILE_64x64, 0.66, 4}};
+    {FM_TILE_PS_128x256_W8, FM_TILE_PP_128x256_W8, 1.00, 2},
+    {FM_TILE_PS_128x128_O2, FM_TILE_128x128_W8, 0.985, 3},
+    {FM_TILE_256x256_W16, -1, 0.96, 1},
+    {FM_TILE_256x128_W8, FM_TILE_256x128_W8, 0.89, 1},
+    {FM_TILE_128x128_W8, FM_TILE_128x128_W8, 0.76, 2},
+    {FM_TILE_64x128, FM_TILE_64x128, 0.71, 3},
+    {FM_TILE_128x64, FM_TILE_128x64, 0.68, 3},
+    {FM_TILE_64x64, FM_TILE_64x64, 0.63, 4}};
 // split-bf16: kernels on prepared slice planes (any shape: the planes are
 // padded to whole tiles)
 // (Strassen, two levels, 16384^3 FP32, i.e. 49 products of 4096^3, whole

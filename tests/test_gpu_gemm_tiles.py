@@ -12,8 +12,8 @@ from cases import make_inputs
 pytestmark = pytest.mark.gpu
 
 DEV = "cuda:0"
-# plain FP64 / FP32 configurations (24-29 are split_bf16 only)
-PLAIN_CFGS = list(range(24)) + [30]
+# plain FP64 / FP32 configurations (24-29 and 31 are split_bf16 only)
+PLAIN_CFGS = list(range(24)) + [30, 32, 33]
 
 
 def _dev(a, fm):

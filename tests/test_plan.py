@@ -22,16 +22,16 @@ def _reset():
 def test_bench_workload_plan():
     # grey424_26_206, 2 levels, 8192^3: one two-level S and T launch, W per
     # level, 676 leaves of 512 x 2048 x 512 = 10816 tiles of 128 x 128
-    # (FmTileCfg 22, two work groups per CU) on 256 CUs: 21 full rounds
+    # (FmTileCfg 32, two work groups per CU) on 256 CUs: 21 full rounds
     # (672 products) + 4 products on 64 x 64 tiles (FmTileCfg 1)
     p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
     assert p["launches"] == {"S": 1, "T": 1, "W": 1, "LEAF": 2, "PEEL": 0}
     assert p["two_level_stages"] == 3
-    assert p["leaf"] == [{"cfg": 22, "products": 672, "tiles": 10752},
+    assert p["leaf"] == [{"cfg": 32, "products": 672, "tiles": 10752},
                          {"cfg": 1, "products": 4, "tiles": 256}]
     fastmm.set_schedule(0, 0, False)
     p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
-    assert p["leaf"] == [{"cfg": 22, "products": 676, "tiles": 10816}]
+    assert p["leaf"] == [{"cfg": 32, "products": 676, "tiles": 10816}]
 
 
 def test_split_workload_plan():
@@ -61,7 +61,7 @@ def test_depth_first_top_level():
     p = fastmm.plan_info("grey424_26_206", 8192, 8192, 8192, 2)
     # level-0 S/T/W plus, per child, its own S/T/W and one leaf launch
     assert p["launches"] == {"S": 27, "T": 27, "W": 27, "LEAF": 26, "PEEL": 0}
-    assert p["leaf"][0] == {"cfg": 22, "products": 26, "tiles": 416}
+    assert p["leaf"][0] == {"cfg": 32, "products": 26, "tiles": 416}
     # level-0 S/T/M blocks plus one child's S/T/M blocks, 8-byte elements
     lvl0 = 22 * 2048 * 4096 + 20 * 4096 * 2048 + 26 * 2048 * 2048
     child = 22 * 512 * 2048 + 20 * 2048 * 512 + 26 * 512 * 512
