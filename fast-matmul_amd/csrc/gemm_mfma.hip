@@ -70,7 +70,7 @@ __device__ __forceinline__ int fm_xcd_remap(int wg, int nwg) {
 // columns (the launch checks), so only the unpredicated 16-byte load path
 // is compiled.
 template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW = 1, int UNR = BK / 4,
-          int PIPE = 0, bool FULL = false>
+          int PIPE = 0, bool FULL = false, bool EVEN = false>
 __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs args) {
   using Op = FmMfma<T>;
   using acc_t = typename Op::acc_t;
@@ -98,7 +98,11 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   // MI355X at 8192^3: FP32 tiles gain 1-4% with global loads; the FP64
   // 128 x 128 two-groups-per-CU tile (the large-product choice) loses 4%
   // (70.8 -> 67.7 TFLOP/s), other FP64 tiles move within +-2%.
-  constexpr bool SPREAD = PIPE == 3 && FULL;
+  // the spread loop runs in whole-tile (FULL) and whole-slice (EVEN: the
+  // launch guarantees K % BK == 0, aligned columns and M % VL == 0 for
+  // every product, so every tile takes 16-byte loads) launches; other
+  // PIPE == 3 launches run the PIPE == 2 loop
+  constexpr bool SPREAD = PIPE == 3 && (FULL || EVEN);
   constexpr int PIPE_EFF = PIPE == 3 ? 2 : PIPE;
   // (the spread loop waits on each part's global load separately: it needs
   // the global address space whatever the element type)
@@ -165,7 +169,7 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   const bool d_aligned = d.a.off % VL == 0 && d.a.ld % VL == 0 && d.b.off % VL == 0 && d.b.ld % VL == 0;
   const bool edge_free = FULL || ((m0 + BM <= M) && (n0 + BN <= N));
   const bool interior =
-      FULL || (CAN_VEC && args.vec && d_aligned && (m0 + BM <= M || M % VL == 0));
+      FULL || EVEN || (CAN_VEC && args.vec && d_aligned && (m0 + BM <= M || M % VL == 0));
 
   // element-load coordinates
   const int a_row = tid % BM, a_k = tid / BM;
@@ -261,7 +265,7 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   // (FULL: K is a positive multiple of BK; the constant keeps a tile's
   // epilogue stores out of the next tile's main loop, where the compiler
   // would otherwise have to wait for every load and store in flight)
-  bool vec_stage = FULL || (interior && BK <= K);
+  bool vec_stage = FULL || EVEN || (interior && BK <= K);
   if (vec_stage) {
     gload_vec(0);
     sstore_vec(0);
@@ -292,6 +296,14 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = Op::mma(bf[set][j], af[set][i], acc[i][j]);
     };
+    // Edge tiles: a vector outside the product loads from the product's
+    // first row (A) or column (B) instead, in bounds, so every lane loads
+    // unconditionally (no branch for the compiler's wait counts to merge
+    // over), and is zeroed when stored to LDS (zeroing it at the load would
+    // make the compiler wait for the load right there).
+    const bool a_ok = edge_free || av_in;
+    const fm_ptr<GM, const T> av_src = a_ok ? av_ptr : A + (int64_t)av_k * lda;
+    auto b_ok = [&](int i) { return edge_free || n0 + bv_col + i * BV_NSTEP < N; };
     auto store_part = [&](int buf, int q) {
 #pragma unroll
       for (int i = 0; i < AV_PER; ++i)
@@ -299,27 +311,31 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
           vec_t v;
 #pragma unroll
           for (int e = 0; e < VL; ++e) v[e] = ra[i * VL + e];
+          if constexpr (!FULL) v = a_ok ? v : vec_t{};
           *reinterpret_cast<vec_t*>(&As[buf][av_k + i * AV_KSTEP][av_row]) = v;
         }
 #pragma unroll
       for (int i = 0; i < BV_PER; ++i)
         if (i % NS4 == q) {
+          const bool ok = FULL || b_ok(i);
 #pragma unroll
-          for (int e = 0; e < VL; ++e) Bs[buf][bv_col + i * BV_NSTEP][bv_k + e] = rb[i * VL + e];
+          for (int e = 0; e < VL; ++e)
+            Bs[buf][bv_col + i * BV_NSTEP][bv_k + e] = ok ? rb[i * VL + e] : T(0);
         }
     };
     auto load_part = [&](int k0, int q) {
 #pragma unroll
       for (int i = 0; i < AV_PER; ++i)
         if (i % NS4 == q) {
-          const vec_t v = *(const fm_ptr<GM, const vec_t>)(av_ptr + (int64_t)(k0 + i * AV_KSTEP) * lda);
+          const vec_t v = *(const fm_ptr<GM, const vec_t>)(av_src + (int64_t)(k0 + i * AV_KSTEP) * lda);
 #pragma unroll
           for (int e = 0; e < VL; ++e) ra[i * VL + e] = v[e];
         }
 #pragma unroll
       for (int i = 0; i < BV_PER; ++i)
         if (i % NS4 == q) {
-          const vec_t v = *(const fm_ptr<GM, const vec_t>)(bv_ptr + k0 + (int64_t)(i * BV_NSTEP) * ldb);
+          const int64_t col = FULL || b_ok(i) ? (int64_t)(i * BV_NSTEP) : -(int64_t)(n0 + bv_col);
+          const vec_t v = *(const fm_ptr<GM, const vec_t>)(bv_ptr + k0 + col * ldb);
 #pragma unroll
           for (int e = 0; e < VL; ++e) rb[i * VL + e] = v[e];
         }
@@ -491,6 +507,13 @@ static void fm_gemm_go(const FmGemmArgs& a, hipStream_t s) {
                          dim3(grid), dim3(NT), 0, s, a);
       return;
     }
+    if constexpr (PIPE == 3) {
+      if (a.even && a.vec) {
+        hipLaunchKernelGGL((fm_gemm_kernel<T, BM, BN, BK, WM, WN, MINW, UNR, PIPE, false, true>),
+                           dim3(grid), dim3(NT), 0, s, a);
+        return;
+      }
+    }
   }
   hipLaunchKernelGGL((fm_gemm_kernel<T, BM, BN, BK, WM, WN, MINW, UNR, PIPE, false>), dim3(grid),
                      dim3(NT), 0, s, a);
@@ -527,6 +550,7 @@ static hipError_t fm_gemm_launch(int cfg, const FmGemmArgs& a, hipStream_t s) {
     case FM_TILE_PI_128x96_O2: fm_gemm_go<T, 128, 96, 16, 2, 2, 2, 4, 2>(a, s); break;
     case FM_TILE_PS_128x128_O2: fm_gemm_go<T, 128, 128, 16, 2, 2, 2, 4, 3>(a, s); break;
     case FM_TILE_PS_128x256_W8: fm_gemm_go<T, 128, 256, 16, 2, 4, 1, 4, 3>(a, s); break;
+    case FM_TILE_PS_128x96_O2: fm_gemm_go<T, 128, 96, 16, 2, 2, 2, 4, 3>(a, s); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

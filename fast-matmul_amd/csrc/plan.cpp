@@ -161,13 +161,15 @@ Launch gemm_launch(std::vector<FmGemmDesc> descs, int cfg, int stage, fm_dtype d
   // unpredicated whole-tile kernel needs every product whole and aligned.
   const int vl = dt == FM_F64 ? 2 : 4;
   const int bk = fm_tile_bk(cfg);
-  bool vec = true, full = true;
+  bool vec = true, full = true, even = true;
   for (const FmGemmDesc& d : descs) {
     vec = vec && d.a.off % vl == 0 && d.a.ld % vl == 0 && d.b.off % vl == 0 && d.b.ld % vl == 0;
     full = full && d.M % bm == 0 && d.N % bn == 0 && d.K % bk == 0 && d.K > 0;
+    even = even && d.M % vl == 0 && d.K % bk == 0 && d.K > 0;
   }
   L.vec_ok = 1;
   L.gm.full = (full && vec) ? 1 : 0;
+  L.gm.even = (even && vec) ? 1 : 0;
   L.cfg = cfg;
   L.gdescs = std::move(descs);
   return L;
@@ -329,8 +331,10 @@ namespace {
 // shape is not offered, its edge kernel spills registers).
 struct TileCand {
   int cfg_full, cfg_edge;
-  double eff;  // throughput per tile area, relative
-  int per_cu;  // resident work groups per CU
+  double eff;           // throughput per tile area, relative
+  int per_cu;           // resident work groups per CU
+  bool spread = false;  // cfg_full also takes edge tiles when every product has whole K
+                        // slices and aligned columns (its EVEN variant)
 };
 // Relative throughput: whole-tile rates on MI355X (8192^3 and the 676-leaf
 // product of 512 x 2048 x 512), best = 1.  Work groups per CU: the lesser of
@@ -346,7 +350,7 @@ const TileCand kF64Tiles[] = {
     {FM_TILE_256x128_W8, FM_TILE_256x128_W8_U1, 0.92, 1},
     {FM_TILE_128x128_W16, FM_TILE_128x128_W16, 0.86, 1},
     {FM_TILE_128x64_W8, FM_TILE_128x64_W8, 0.84, 2},
-    {FM_TILE_PI_128x96_O2, FM_TILE_PI_128x96_O2, 0.84, 2},
+    {FM_TILE_PS_128x96_O2, FM_TILE_PI_128x96_O2, 0.87, 2, true},
     {FM_TILE_64x64, FM_TILE_64x64, 0.75, 3}};
 const TileCand kF32Tiles[] = {
     {FM_TILE_PS_128x256_W8, FM_TILE_PP_128x256_W8, 1.00, 2},
@@ -367,8 +371,15 @@ const TileCand kSplitTiles[] = {{FM_TILE_SP_128x256_BD, FM_TILE_SP_128x256_BD, 1
                                 {FM_TILE_SP_128x256_IL, FM_TILE_SP_128x256_IL, 0.92, 1},
                                 {FM_TILE_SP_128x128, FM_TILE_SP_128x128, 0.88, 2},
                                 {FM_TILE_SP_128x128_DB, FM_TILE_SP_128x128_DB, 0.83, 1}};
-// edge kernels run slower than whole-tile ones (predicated loads)
+// edge kernels run slower than whole-tile ones (predicated loads).  The
+// spread kernels' edge variant is offered for the 128 x 96 tile only: on
+// 128 x 128 and 128 x 256 tiles it measured 6% slower per tile than the
+// whole-tile kernel (classical 131072 x 2400 x 2400 FP64 66.8 TFLOP/s as
+// one launch, 71.8 cut into a whole-tile core and a strip; the C3 FP32
+// batch 119.2 against 124.4), while as the tall-skinny leaves' 88-column
+// strip kernel it took the one-level multiply from 68.7 to 71.0
 const double kEdgePenalty = 0.93;
+const double kSpreadEdgePenalty = 0.93;
 
 const TileCand* tile_cands(fm_dtype dt, fm_mode mode, int* n) {
   if (mode == FM_MODE_SPLIT_BF16) {
@@ -405,14 +416,42 @@ bool whole_tiles(fm_dtype dt, int cfg, const std::vector<FmGemmDesc>& prods) {
   return true;
 }
 
+// Every product has whole K slices, 16-byte aligned columns and M a
+// multiple of the vector width (the spread kernels' EVEN launch).
+bool even_slices(fm_dtype dt, int cfg, const std::vector<FmGemmDesc>& prods) {
+  const int bk = fm_tile_bk(cfg), vl = dt == FM_F64 ? 2 : 4;
+  for (const FmGemmDesc& d : prods)
+    if (d.M % vl || d.K % bk || d.K == 0 || d.a.off % vl || d.a.ld % vl || d.b.off % vl ||
+        d.b.ld % vl)
+      return false;
+  return true;
+}
+
+// How a candidate takes a product list: the kernel (-1: not offered) and
+// its form, 2 whole tiles, 1 the spread kernel's edge variant, 0 the
+// predicated edge kernel.
+int cand_kernel(fm_dtype dt, const TileCand& c, const std::vector<FmGemmDesc>& prods, int* form) {
+  if (whole_tiles(dt, c.cfg_full, prods)) {
+    *form = 2;
+    return c.cfg_full;
+  }
+  if (c.spread && even_slices(dt, c.cfg_full, prods)) {
+    *form = 1;
+    return c.cfg_full;
+  }
+  *form = 0;
+  return c.cfg_edge;
+}
+
 // Estimated time of one launch of `tiles` tiles: rounds of per_cu work
 // groups on each of the 256 CUs.
-double tile_time(const TileCand& c, int64_t tiles, bool full) {
+double tile_time(const TileCand& c, int64_t tiles, int form) {
   int bm, bn;
   fm_tile_dims(c.cfg_full, &bm, &bn);
   const int64_t slots = 256LL * c.per_cu;
   const double rounds = (double)((tiles + slots - 1) / slots);
-  return rounds * bm * bn * c.per_cu / (c.eff * (full ? 1.0 : kEdgePenalty));
+  const double pen = form == 2 ? 1.0 : form == 1 ? kSpreadEdgePenalty : kEdgePenalty;
+  return rounds * bm * bn * c.per_cu / (c.eff * pen);
 }
 
 const TileCand* find_cand(fm_dtype dt, fm_mode mode, int cfg) {
@@ -430,10 +469,10 @@ int best_tile(fm_dtype dt, fm_mode mode, const std::vector<FmGemmDesc>& prods, d
   int best = FM_TILE_64x64;
   double best_t = 1e300;
   for (int i = 0; i < nc; ++i) {
-    const bool full = whole_tiles(dt, cands[i].cfg_full, prods);
-    const int cfg = full ? cands[i].cfg_full : cands[i].cfg_edge;
+    int form;
+    const int cfg = cand_kernel(dt, cands[i], prods, &form);
     if (cfg < 0) continue;
-    const double t = tile_time(cands[i], count_tiles(cfg, prods), full);
+    const double t = tile_time(cands[i], count_tiles(cfg, prods), form);
     if (t < best_t * 0.999) {
       best_t = t;
       best = cfg;
@@ -484,8 +523,10 @@ bool carve_full(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods, std::vec
   double t_strips = 0, t_odd = 0;
   const int cs = st.empty() ? cfg : best_tile(P.dtype, P.mode, st, &t_strips);
   const int co = od.empty() ? cfg : best_tile(P.dtype, P.mode, od, &t_odd);
-  const double t_core = tile_time(*c, count_tiles(c->cfg_full, core), true);
-  const double t_all = tile_time(*c, count_tiles(cfg, prods), false);
+  int form_all;
+  (void)cand_kernel(P.dtype, *c, prods, &form_all);
+  const double t_core = tile_time(*c, count_tiles(c->cfg_full, core), 2);
+  const double t_all = tile_time(*c, count_tiles(cfg, prods), form_all);
   if (t_core + t_strips + t_odd >= 0.97 * t_all) return false;
   prods = std::move(core);
   strips = std::move(st);
@@ -533,16 +574,17 @@ void split_remainder(const Plan& P, int cfg, std::vector<FmGemmDesc>& prods,
   std::vector<FmGemmDesc> head(prods.begin(), prods.begin() + (std::ptrdiff_t)cut);
   double t_tail = 0;
   const int cr = best_tile(P.dtype, P.mode, tail, &t_tail);
-  const bool whole = whole_tiles(P.dtype, c->cfg_full, prods);
-  const double t_one = tile_time(*c, total, whole);
-  const double t_two = tile_time(*c, full, whole_tiles(P.dtype, c->cfg_full, head)) + t_tail;
+  int form_one, form_head;
+  (void)cand_kernel(P.dtype, *c, prods, &form_one);
+  const int ch = cand_kernel(P.dtype, *c, head, &form_head);
+  const double t_one = tile_time(*c, total, form_one);
+  const double t_two = tile_time(*c, full, form_head) + t_tail;
   if (t_two >= t_one * 0.99) return;
   prods.resize(cut);
   rest = std::move(tail);
   *cfg_rest = cr;
   // the full rounds may be whole tiles even where the remainder was not
-  if (c->cfg_edge >= 0 || whole_tiles(P.dtype, c->cfg_full, prods))
-    *cfg_head = whole_tiles(P.dtype, c->cfg_full, prods) ? c->cfg_full : c->cfg_edge;
+  if (ch >= 0) *cfg_head = ch;
 }
 
 // Depth-first top level (the reference's DFS order, algorithms/strassen.hpp:

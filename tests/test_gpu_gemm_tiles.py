@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 DEV = "cuda:0"
 # plain FP64 / FP32 configurations (24-29 and 31 are split_bf16 only)
-PLAIN_CFGS = list(range(24)) + [30, 32, 33]
+PLAIN_CFGS = list(range(24)) + [30, 32, 33, 34]
 
 
 def _dev(a, fm):
@@ -30,9 +30,11 @@ def _reset(fm):
 @pytest.mark.parametrize("cfg", PLAIN_CFGS)
 def test_tile_config_exact_on_integers(fm, cfg, dtype):
     fm.set_gemm_tile(cfg)
-    # whole-tile shapes (every configuration's full-tile variant) and edges
+    # whole-tile shapes (every configuration's full-tile variant), edges, and
+    # edges with whole K slices (the spread kernels' edge variant)
     for shape in [(1, 1, 1), (17, 33, 65), (128, 128, 128), (256, 256, 256), (512, 160, 512),
-                  (200, 300, 150), (257, 129, 383), (64, 1000, 48)]:
+                  (200, 300, 150), (257, 129, 383), (64, 1000, 48), (200, 160, 150),
+                  (130, 64, 260)]:
         m, k, n = shape
         A, B, C0 = make_inputs(m, k, n, 31, "int", dtype, with_c=True)
         exact = A.astype(np.float64) @ B.astype(np.float64)
@@ -40,7 +42,7 @@ def test_tile_config_exact_on_integers(fm, cfg, dtype):
         assert np.array_equal(C.astype(np.float64), 2 * exact - 3 * C0.astype(np.float64)), shape
 
 
-@pytest.mark.parametrize("shape", [(515, 1031, 263), (512, 1024, 512)])
+@pytest.mark.parametrize("shape", [(515, 1031, 263), (512, 1024, 512), (204, 512, 190)])
 @pytest.mark.parametrize("dtype,tol", [(np.float64, 4e-16), (np.float32, 2e-7)])
 @pytest.mark.parametrize("cfg", PLAIN_CFGS)
 def test_tile_config_random(fm, cfg, dtype, tol, shape):

@@ -109,13 +109,44 @@ def test_remainder_split_bit_identical(fm):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_whole_tile_carve_out_bit_identical(fm, dtype):
-    # strassen L=1 on 8200 x 1024 x 4340: 7 leaves of 4100 x 512 x 2170, each
-    # cut into a whole-tile core and row / column strips, against one
+    # classical 8191 x 512 x 4100 (FP32: 12286 x 512 x 4100) with an aligned
+    # leading dimension: M is not a multiple of the 16-byte vector width, so
+    # the spread kernel's edge variant cannot take it and the product is cut
+    # into a whole-tile core and row / column strips, against one
     # edge-kernel launch
-    m, k, n = 8200, 1024, 4340
+    vl = 2 if dtype == np.float64 else 4
+    m = 8191 if dtype == np.float64 else 12286
+    k, n = 512, 4100
     A, B, _ = make_inputs(m, k, n, 25, "u11", dtype)
-    Ad, Bd = _dev(A, fm), _dev(B, fm)
-    Cs, ps = _run(fm, "strassen", Ad, Bd, None, 1, (0, 0, True))
-    Cu, pu = _run(fm, "strassen", Ad, Bd, None, 1, (0, 0, False))
+    pad = np.zeros((m + vl - m % vl, k), dtype=dtype)
+    pad[:m] = A
+    Ad = _dev(pad, fm)[:m]
+    assert Ad.stride(1) % vl == 0
+    Bd = _dev(B, fm)
+    Cs, ps = _run(fm, "strassen", Ad, Bd, None, 0, (0, 0, True))
+    Cu, pu = _run(fm, "strassen", Ad, Bd, None, 0, (0, 0, False))
     assert (ps["LEAF"][1], pu["LEAF"][1]) == (2, 1)
     assert torch.equal(Cs, Cu)
+
+
+@pytest.mark.parametrize("tiles", [(34, 30), (32, 16), (33, 19)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_spread_edge_launch_bit_identical(fm, dtype, tiles):
+    # strassen L=1 on 4200 x 512 x 4200: 7 leaves of 2100 x 256 x 2100 with
+    # whole K slices and aligned columns but partial tiles, on a spread
+    # tile forced (its edge variant: 128 x 96, 128 x 128, 128 x 256) against
+    # a predicated edge kernel of the same tile forced: the same bits
+    spread, edge = tiles
+    m, k, n = 4200, 512, 4200
+    A, B, _ = make_inputs(m, k, n, 25, "u11", dtype)
+    Ad, Bd = _dev(A, fm), _dev(B, fm)
+    out = []
+    try:
+        for cfg in (spread, edge):
+            fm.set_gemm_tile(cfg)
+            C, prof = _run(fm, "strassen", Ad, Bd, None, 1, (0, 0, True))
+            assert prof["LEAF"][1] == 1
+            out.append(C)
+    finally:
+        fm.set_gemm_tile(-1)
+    assert torch.equal(out[0], out[1])
