@@ -34,6 +34,10 @@ OPTS = {
     "tile64": dict(gemm_tile=1),
     "tile128x64_w8": dict(gemm_tile=12),
     "tile128x96": dict(gemm_tile=30),
+    # the spread kernels: whole-tile, whole-slice edge and predicated variants
+    "tile_spread128": dict(gemm_tile=32),
+    "tile_spread128x256": dict(gemm_tile=33),
+    "tile_spread128x96": dict(gemm_tile=34),
 }
 
 
