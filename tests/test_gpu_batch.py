@@ -85,6 +85,27 @@ def test_batch_autotuner_choices_and_fp32_accuracy(fm, oracle_mod):
 
 
 
+def test_batch_autotuner_picks_fast_form_where_it_pays(fm, oracle_mod):
+    # FP64: one 8192^3 problem, where a fast algorithm's saved products
+    # outweigh its stage bytes on this GPU, among 512^3 ones, where they do
+    # not: the cost model picks a fast form for the large problem only
+    shapes = [(8192, 8192, 8192)] + [(512, 512, 512)] * 3
+    probs, host = _problems(fm, shapes, torch.float64, "u11", 5)
+    b = fm.Batch(probs, graph=False)
+    b.run()
+    torch.cuda.synchronize()
+    chosen = [b.choice(i) for i in range(len(probs))]
+    assert chosen[0][0] != "classical" and chosen[0][1] >= 1, chosen[0]
+    assert all(c == ("classical", 0) for c in chosen[1:]), chosen
+    rng = np.random.default_rng(6)
+    for (p, (A, B, C0), (alg, st)) in zip(probs, host, chosen):
+        m, n = A.shape[0], B.shape[1]
+        rows, cols = rng.integers(0, m, 128), rng.integers(0, n, 128)
+        q, absab = oracle_mod.quad_entries(A, B, rows, cols)
+        got = p["C"].cpu().numpy()[rows, cols]
+        assert np.all(np.abs(got - q) <= 64 * (1 + st) * 2.0 ** -53 * absab), (alg, st)
+
+
 def test_batch_split_planes_match_single_calls(fm):
     """split_bf16 batches: one preparation launch for every problem's
     operands, one grouped plane launch.  The plane kernels compute the
