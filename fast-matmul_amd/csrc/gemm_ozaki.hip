@@ -426,6 +426,142 @@ __global__ void __launch_bounds__(256, 2) fm_ozaki_gemm32(FmOzakiArgs args) {
   }
 }
 
+// ---- grouped digit product, 64 x 128 tiles -------------------------------------
+// fm_ozaki_gemm32 with each wave on a 32 x 64 block: two B fragment columns
+// share every A fragment, so a wave issues 72 MFMAs per 32-deep slice for
+// 8 A and 16 B fragments (fm_ozaki_gemm32: 36 for 8 and 8).  Global and L2
+// traffic per MFMA drops to a little over half; the 2 x 8 accumulator sets
+// (256 registers per lane) leave one work group of four waves per CU.  B
+// digit planes are padded to 128 columns for it.  Same per-scale int32
+// sums and FP64 reconstruction as fm_ozaki_gemm32, so the result is the
+// same bit for bit.
+template <int S>
+__global__ void __launch_bounds__(256, 1) fm_ozaki_gemm_w(FmOzakiArgs args) {
+  constexpr int KS = 2;
+  constexpr int BM = 64, BN = 128, TN = 2, J = 2 * S, JS = KS * J, GROUP = 8;
+  constexpr int NCH = (J * BM + 255) / 256;  // per 32-deep slice
+  __shared__ __align__(16) fm_i32x4 As[2][JS][BM];
+
+  int wg = blockIdx.x;
+  {
+    const int nwg = gridDim.x, full = nwg & ~7;
+    if (wg < full) wg = (wg & 7) * (full >> 3) + (wg >> 3);
+  }
+  int lo = 0, hi = args.ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (args.descs[mid].tile_begin <= wg)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const FmOzakiDesc d = args.descs[lo];
+  const int local = wg - d.tile_begin;
+  const int per_group = GROUP * d.tiles_n, grp = local / per_group;
+  const int first = grp * GROUP, rows_g = min(GROUP, d.tiles_m - first);
+  const int rem = local - grp * per_group;
+  const int m0 = (first + rem % rows_g) * BM, n0 = (rem / rows_g) * BN;
+  const fm_gptr<char> ws = fm_global(reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]));
+  const fm_gptr<const fm_i32x4> sa = (fm_gptr<const fm_i32x4>)(ws + d.a_slc) + m0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int fr = lane & 31, fh = lane >> 5;
+  const int64_t pa = d.rows_pad, pb = d.cols_pad;
+  // this lane's B columns (n0 + wn 64 + 32 j + fr < cols_pad) and K half
+  const fm_gptr<const fm_i32x4> sbl =
+      (fm_gptr<const fm_i32x4>)(ws + d.b_slc) + n0 + wn * 64 + fr + fh * pb;
+
+  fm_i32x16 acc[TN][S];
+#pragma unroll
+  for (int j = 0; j < TN; ++j)
+#pragma unroll
+    for (int u = 0; u < S; ++u)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[j][u][e] = 0;
+
+  fm_i32x4 ra[NCH];
+  auto gload = [&](int kc, int h) {
+#pragma unroll
+    for (int u = 0; u < NCH; ++u) {
+      const int o = tid + u * 256, row = o & (BM - 1), jj = o / BM;
+      if ((J * BM) % 256 == 0 || o < J * BM) ra[u] = sa[(int64_t)(kc * JS + h * J + jj) * pa + row];
+    }
+  };
+  auto sstore = [&](int buf, int h) {
+#pragma unroll
+    for (int u = 0; u < NCH; ++u) {
+      const int o = tid + u * 256, row = o & (BM - 1), jj = o / BM;
+      if ((J * BM) % 256 == 0 || o < J * BM) As[buf][h * J + jj][row] = ra[u];
+    }
+  };
+
+  const int nkc = d.Kp / (32 * KS);  // Kp is a multiple of 64
+  fm_i32x4 bf[TN][S];
+  if (nkc > 0) {
+#pragma unroll
+    for (int h = 0; h < KS; ++h) {
+      gload(0, h);
+      sstore(0, h);
+    }
+#pragma unroll
+    for (int t = 0; t < S; ++t)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bf[j][t] = sbl[(int64_t)(2 * t) * pb + 32 * j];
+  }
+  __syncthreads();
+  for (int kc = 0; kc < nkc; ++kc) {
+    const int buf = kc & 1;
+    // as in fm_ozaki_gemm32: the last stage reloads itself rather than
+    // branching, so no branch join makes the compiler wait for every load
+    const int kn = min(kc + 1, nkc - 1);
+#pragma unroll
+    for (int h = 0; h < KS; ++h) {
+      gload(kn, h);
+      __builtin_amdgcn_sched_barrier(0);
+      const int64_t nb = (int64_t)((h + 1 < KS ? kc * KS + h + 1 : kn * KS) * J) * pb;
+      fm_i32x4 af[S];
+#pragma unroll
+      for (int s = 0; s < S; ++s) af[s] = As[buf][h * J + 2 * s + fh][wm * 32 + fr];
+#pragma unroll
+      for (int t = S - 1; t >= 0; --t) {
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+#pragma unroll
+          for (int s = 0; s + t < S; ++s)
+            acc[j][s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(bf[j][t], af[s], acc[j][s + t], 0, 0, 0);
+          bf[j][t] = sbl[nb + (int64_t)(2 * t) * pb + 32 * j];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      sstore(buf ^ 1, h);
+    }
+    __syncthreads();
+  }
+
+  const fm_gptr<const int32_t> ea = (fm_gptr<const int32_t>)(ws + d.a_exp);
+  const fm_gptr<const int32_t> eb = (fm_gptr<const int32_t>)(ws + d.b_exp);
+  const fm_gptr<double> C = fm_global(reinterpret_cast<double*>(fm_base(args.bases, d.c.role)) + d.c.off);
+  const int64_t ldc = d.c.ld;
+  const int row = m0 + wm * 32 + fr;
+  if (row >= d.M) return;
+  const int er = ea[row];
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+      const int col = n0 + wn * 64 + 32 * j + (e & 3) + 4 * fh + 8 * (e >> 2);
+      if (col >= d.N) continue;
+      double sum = 0.0;
+#pragma unroll
+      for (int u = S - 1; u >= 0; --u) sum = __dadd_rn(sum, ldexp((double)acc[j][u][e], -7 * (u + 2)));
+      double r = __dmul_rn(ldexp(sum, er + eb[col]), d.alpha);
+      const fm_gptr<double> p = C + row + (int64_t)col * ldc;
+      if (d.beta != 0.0) r = __dadd_rn(r, __dmul_rn(d.beta, *p));
+      *p = r;
+    }
+  }
+}
+
 template <int S>
 static hipError_t fm_ozaki_run(const FmOzakiArgs& a, int max_rows, int max_cols, int phases,
                                hipStream_t s) {
@@ -438,8 +574,10 @@ static hipError_t fm_ozaki_run(const FmOzakiArgs& a, int max_rows, int max_cols,
   if (!(phases & FM_PHASE_PRODUCT)) return hipGetLastError();
   if (a.variant == 0)
     hipLaunchKernelGGL((fm_ozaki_gemm<S>), dim3(a.total_tiles), dim3(256), 0, s, a);
-  else
+  else if (a.variant == 1)
     hipLaunchKernelGGL((fm_ozaki_gemm32<S>), dim3(a.total_tiles), dim3(256), 0, s, a);
+  else
+    hipLaunchKernelGGL((fm_ozaki_gemm_w<S>), dim3(a.total_tiles), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

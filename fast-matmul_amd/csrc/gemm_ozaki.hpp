@@ -11,7 +11,7 @@
 struct FmOzakiDesc {
   FmOp a, b, c;                 // FP64 operands
   int32_t M, N, K, Kp;          // Kp: K rounded up to 64 (digit row length)
-  int32_t rows_pad, cols_pad;   // M, N rounded up to 64
+  int32_t rows_pad, cols_pad;   // M rounded up to 64, N to the tile width (64 or 128)
   int32_t tiles_m, tiles_n, tile_begin, pad;
   int64_t a_slc, b_slc;         // byte offsets of the digit planes in the workspace
   int64_t a_exp, b_exp;         // byte offsets of the int32 row / column exponents
@@ -23,7 +23,8 @@ struct FmOzakiArgs {
   const FmOzakiDesc* descs;
   int32_t ndesc;
   int32_t total_tiles;
-  int32_t variant;  // 0: 16 x 16 x 64 MFMA kernel, 64-deep slices; 1: 32 x 32 x 32, two 32-deep slices per stage
+  int32_t variant;  // 0: 16 x 16 x 64 MFMA kernel, 64-deep slices; 1: 32 x 32 x 32, two 32-deep
+                    // slices per stage, 64 x 64 tiles; 2: as 1 on 64 x 128 tiles
   int32_t pad;
 };
 

@@ -182,6 +182,11 @@ fm_status ozaki_launch(int S, int tile, const std::vector<FmGemmDesc>& prods, si
   int32_t t = 0;
   int max_rows = 0, max_cols = 0;
   size_t off = (size_t)align_up((int64_t)*ws_bytes, 256);
+  // kernel: FmTileCfg 64x64 forced selects the 16 x 16 x 64 one, 128x64
+  // the 64 x 64-tile 32 x 32 x 32 one; otherwise 64 x 128 tiles of
+  // 32 x 32 x 32 products (B digit planes padded to 128 columns)
+  const int variant = tile == FM_TILE_64x64 ? 0 : tile == FM_TILE_128x64 ? 1 : 2;
+  const int bn = variant == 2 ? 128 : 64;
   for (const FmGemmDesc& g : prods) {
     FmOzakiDesc d;
     memset(&d, 0, sizeof(d));
@@ -195,9 +200,9 @@ fm_status ozaki_launch(int S, int tile, const std::vector<FmGemmDesc>& prods, si
     if (d.Kp > fm_ozaki_max_kp(S))
       return set_err(FM_ERR_SHAPE, "ozaki_int8: leaf K too large for exact int32 sums");
     d.rows_pad = (int32_t)align_up(g.M, 64);
-    d.cols_pad = (int32_t)align_up(g.N, 64);
+    d.cols_pad = (int32_t)align_up(g.N, bn);
     d.tiles_m = d.rows_pad / 64;
-    d.tiles_n = d.cols_pad / 64;
+    d.tiles_n = d.cols_pad / bn;
     d.tile_begin = t;
     t += d.tiles_m * d.tiles_n;
     d.alpha = g.alpha;
@@ -218,8 +223,7 @@ fm_status ozaki_launch(int S, int tile, const std::vector<FmGemmDesc>& prods, si
   L.stage = FM_STAGE_LEAF;
   L.oz.ndesc = (int32_t)L.odescs.size();
   L.oz.total_tiles = t;
-  // the 64 x 64 tile override selects the 16 x 16 x 64 kernel
-  L.oz.variant = tile == FM_TILE_64x64 ? 0 : 1;
+  L.oz.variant = variant;
   L.oz_slices = S;
   L.oz_rows = max_rows;
   L.oz_cols = max_cols;

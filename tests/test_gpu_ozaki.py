@@ -24,9 +24,10 @@ def _dev(a, fm):
     return fm.col_major(torch.from_numpy(np.asfortranarray(a)).to(DEV))
 
 
-# FmTileCfg 1 (64 x 64) selects the 16 x 16 x 64 digit kernel, -1 the
-# default 32 x 32 x 32 one
-@pytest.mark.parametrize("kernel", [-1, 1])
+# FmTileCfg 1 (64 x 64) selects the 16 x 16 x 64 digit kernel, 2 (128 x 64)
+# the 32 x 32 x 32 one on 64 x 64 tiles, -1 the default 32 x 32 x 32 one on
+# 64 x 128 tiles
+@pytest.mark.parametrize("kernel", [-1, 1, 2])
 @pytest.mark.parametrize("S", [6, 7, 8])
 @pytest.mark.parametrize("shape", [(64, 64, 64), (100, 130, 70), (257, 500, 55), (1, 64, 1),
                                    (130, 700, 96)])
