@@ -128,9 +128,15 @@ struct FmSplitPrep {
   int32_t rows, K;
   int32_t rows_pad, Kp;  // rows rounded up to the tile, K rounded up to 32
   int32_t kcontig;       // 1: K contiguous in the source (B), 0: rows contiguous (A)
-  int32_t pad;
+  int32_t wg_begin;      // first work group of this operand in the preparation launch
   int64_t dst;           // byte offset in the workspace
 };
+
+// K slices of 32 per preparation work group (64 rows each)
+constexpr int FM_PREP_KCH = 4;
+inline int fm_prep_wgs(const FmSplitPrep& p) {
+  return (p.rows_pad / 64) * ((p.Kp / 32 + FM_PREP_KCH - 1) / FM_PREP_KCH);
+}
 
 // One product C = alpha * A B + beta * C from the planes of its operands.
 struct FmSplitDesc {
@@ -148,7 +154,7 @@ struct FmSplitArgs {
   int32_t ndesc, total_tiles;
   int32_t uniform, grid;        // as in FmGemmArgs
   int32_t nprep;
-  int32_t prep_bx, prep_by;     // prep grid: row blocks of 64, K slices of 32
+  int32_t prep_wgs;             // work groups of the preparation launch (all operands)
   int32_t even_k;               // 1: every product has an even number of K slices
 };
 

@@ -256,19 +256,26 @@ hipError_t fm_gemm_launch_bf16x3(int cfg, const FmGemmArgs& a, hipStream_t s) {
 // One work group cuts 64 rows x FM_PREP_KCH consecutive 32-deep K slices of
 // one operand: per slice each thread 8 consecutive k of one row, the 4
 // threads of a row in adjacent lanes, so a wave writes 16 consecutive
-// 192-byte row chunks.  Sources with rows contiguous (A) load 64-byte runs
-// down each column; sources with K contiguous (B) 128-byte runs along each
-// row.  The grid covers the largest operand of the launch; several slices
-// per work group spread its descriptor load over more work and keep the
-// grid small when the operands differ in size (a work group past its
-// operand's end exits after that load).
-constexpr int FM_PREP_KCH = 4;
-
-__global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first) {
-  const FmSplitPrep d = args.preps[first + blockIdx.z];
-  const int r0 = blockIdx.x * 64, kc0 = blockIdx.y * FM_PREP_KCH;
-  const int nkc = d.Kp >> 5;
-  if (r0 >= d.rows_pad || kc0 >= nkc) return;
+// 192-byte row chunks.  The launch is flat: work group b belongs to the
+// operand whose wg_begin is the largest one <= b (binary search), so no
+// work group of an uneven batch is empty.  Sources with K contiguous (B)
+// load 128-byte runs along each row; sources with rows contiguous (A) go
+// through LDS: each wave loads 256-byte runs down 64 rows of a column, and
+// every thread then reads its row's 8 k across the staged slice.
+__global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args) {
+  int lo = 0, hi = args.nprep - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (args.preps[mid].wg_begin <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const FmSplitPrep d = args.preps[lo];
+  const int nkc = d.Kp >> 5, nkb = (nkc + FM_PREP_KCH - 1) / FM_PREP_KCH;
+  const int local = b - d.wg_begin, rb = local / nkb;
+  const int r0 = rb * 64, kc0 = (local - rb * nkb) * FM_PREP_KCH;
   const fm_gptr<const float> __restrict__ src =
       fm_global(reinterpret_cast<const float*>(fm_base(args.bases, d.src.role)) + d.src.off);
   const fm_gptr<char> ws = fm_global(reinterpret_cast<char*>(args.bases.p[FM_ROLE_WS]));
@@ -278,6 +285,12 @@ __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first
   const bool rok = row < d.rows;
   const int64_t ld = d.src.ld;
   const int kc1 = min(kc0 + FM_PREP_KCH, nkc);
+  // [k][row] of one slice (A sources); rows padded to 66 words so that the
+  // reads below (16 rows of 4 k groups per wave, 8 k apart) hit 64 banks
+  __shared__ float stage[32][64 + 2];
+  // A sources: thread t loads row t % 64 of columns t / 64 + 4 i
+  const int lr = tid & 63, lk = tid >> 6;
+  const bool lrok = r0 + lr < d.rows;
   for (int kc = kc0; kc < kc1; ++kc) {
     const int k = kc * 32 + 8 * q;
     float x[8];
@@ -286,9 +299,16 @@ __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args, int first
 #pragma unroll
       for (int e = 0; e < 8; ++e) x[e] = (rok && k + e < d.K) ? p[e] : 0.f;
     } else {
-      const fm_gptr<const float> p = src + row + (int64_t)k * ld;
+      const fm_gptr<const float> p = src + (r0 + lr) + (int64_t)(kc * 32 + lk) * ld;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) x[e] = (rok && k + e < d.K) ? p[(int64_t)e * ld] : 0.f;
+      for (int i = 0; i < 8; ++i) {
+        const int kk = lk + 4 * i;
+        stage[kk][lr] = (lrok && kc * 32 + kk < d.K) ? p[(int64_t)(4 * i) * ld] : 0.f;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < 8; ++e) x[e] = stage[8 * q + e][r];
+      __syncthreads();  // the next slice overwrites the stage
     }
     uint4 sl[3];
     fm_split3(x, sl);
@@ -650,13 +670,8 @@ __global__ void __launch_bounds__(64 * WM * WN) fm_split_gemm_bd(FmSplitArgs arg
 }
 
 hipError_t fm_split_launch(int cfg, const FmSplitArgs& a, int phases, hipStream_t s) {
-  if ((phases & FM_PHASE_PREP) && a.nprep > 0 && a.prep_bx > 0 && a.prep_by > 0) {
-    for (int first = 0; first < a.nprep; first += 65535) {
-      const int nz = a.nprep - first < 65535 ? a.nprep - first : 65535;
-      hipLaunchKernelGGL(fm_split_prep,
-                         dim3(a.prep_bx, (a.prep_by + FM_PREP_KCH - 1) / FM_PREP_KCH, nz),
-                         dim3(256), 0, s, a, first);
-    }
+  if ((phases & FM_PHASE_PREP) && a.nprep > 0 && a.prep_wgs > 0) {
+    hipLaunchKernelGGL(fm_split_prep, dim3(a.prep_wgs), dim3(256), 0, s, a);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

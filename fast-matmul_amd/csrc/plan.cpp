@@ -243,7 +243,7 @@ Launch split_launch(std::vector<FmGemmDesc> prods, int cfg, size_t* ws_bytes) {
   L.cfg = cfg;
   size_t off = (size_t)align_up((int64_t)*ws_bytes, 256);
   std::map<std::tuple<int, int64_t, int64_t, int, int, int>, int64_t> seen;
-  int max_rb = 0, max_kc = 0;
+  int32_t prep_wgs = 0;
   auto planes = [&](const FmOp& src, int rows, int K, int pad_to, int kcontig) {
     const auto key = std::make_tuple(src.role, src.off, src.ld, rows, K, kcontig);
     auto it = seen.find(key);
@@ -258,8 +258,8 @@ Launch split_launch(std::vector<FmGemmDesc> prods, int cfg, size_t* ws_bytes) {
     r.kcontig = kcontig;
     r.dst = (int64_t)off;
     off += (size_t)align_up((int64_t)r.rows_pad * r.Kp * 6, 256);
-    max_rb = std::max(max_rb, r.rows_pad / 64);
-    max_kc = std::max(max_kc, r.Kp / 32);
+    r.wg_begin = prep_wgs;
+    prep_wgs += fm_prep_wgs(r);
     L.spreps.push_back(r);
     seen.emplace(key, r.dst);
     return r.dst;
@@ -289,8 +289,7 @@ Launch split_launch(std::vector<FmGemmDesc> prods, int cfg, size_t* ws_bytes) {
   L.sp.total_tiles = t;
   L.sp.uniform = uniform ? 1 : 0;
   L.sp.nprep = (int32_t)L.spreps.size();
-  L.sp.prep_bx = max_rb;
-  L.sp.prep_by = max_kc;
+  L.sp.prep_wgs = prep_wgs;
   L.sp.even_k = 1;
   for (const FmSplitDesc& d : L.sdescs)
     if ((d.Kp / 32) % 2 != 0) L.sp.even_k = 0;
