@@ -19,6 +19,7 @@
 #include <type_traits>
 
 #include "alg_desc.hpp"
+#include "gmem.hpp"
 
 __device__ __forceinline__ double fm_add(double a, double b) { return __dadd_rn(a, b); }
 __device__ __forceinline__ double fm_sub(double a, double b) { return __dadd_rn(a, -b); }
@@ -190,10 +191,10 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2(FmStageArgs a) 
   const int q = blockIdx.y;
   const FmNodeView nv = fm_node_view(a, q, R * NOUT, R * NV);
   const FmOp op = nv.op;
-  const T* src = reinterpret_cast<const T*>(fm_base(a.bases, op.role)) + op.off;
-  T* ws = reinterpret_cast<T*>(a.bases.p[FM_ROLE_WS]);
-  T* dst = ws + nv.out;
-  T* vdst = ws + nv.view;
+  const fm_gptr<const T> src = fm_global(reinterpret_cast<const T*>(fm_base(a.bases, op.role)) + op.off);
+  const fm_gptr<T> ws = fm_global(reinterpret_cast<T*>(a.bases.p[FM_ROLE_WS]));
+  const fm_gptr<T> dst = ws + nv.out;
+  const fm_gptr<T> vdst = ws + nv.view;
   const uint32_t rows = (uint32_t)nv.rows2;
   const uint32_t total = rows * (uint32_t)nv.cols2;
   const int64_t ld = op.ld, brows = nv.rows, bcols = nv.cols, brows2 = nv.rows2,
@@ -204,14 +205,18 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2(FmStageArgs a) 
     const uint32_t j = e / rows;
     const uint32_t i = e - j * rows;
     T v[NB][NB];  // v[block][sub-block]
+    // one lane address; each load adds its (lane-uniform) block and
+    // sub-block offsets, a block's loads at a time, so that the 64-bit
+    // addresses of all NB * NB loads are not live together
+    const fm_gptr<const T> lsrc = src + i + (int64_t)j * ld;
 #pragma unroll
-    for (int b = 0; b < NB; ++b)
+    for (int b = 0; b < NB; ++b) {
+      const fm_gptr<const T> bsrc = lsrc + (int64_t)(b / BC) * brows + (int64_t)(b % BC) * bcols * ld;
 #pragma unroll
-      for (int sb = 0; sb < NB; ++sb) {
-        const int64_t r0 = (int64_t)(b / BC) * brows + (int64_t)(sb / BC) * brows2 + i;
-        const int64_t c0 = (int64_t)(b % BC) * bcols + (int64_t)(sb % BC) * bcols2 + j;
-        v[b][sb] = src[r0 + c0 * ld];
-      }
+      for (int sb = 0; sb < NB; ++sb)
+        v[b][sb] = bsrc[(int64_t)(sb / BC) * brows2 + (int64_t)(sb % BC) * bcols2 * ld];
+      __builtin_amdgcn_sched_barrier(0);
+    }
     const int64_t opos = i + (int64_t)j * nv.out_ld;
     auto product = [&](auto rc) {
       constexpr int r = decltype(rc)::value;
@@ -237,8 +242,14 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2(FmStageArgs a) 
         Cmb::s_combine(par, out, x);
       else
         Cmb::t_combine(par, out, x);
+      // one running address per product, advanced a slab per output: the
+      // outputs' slab offsets are not all kept live at once
+      fm_gptr<T> op = dst + (int64_t)r * NOUT * slab + opos;
 #pragma unroll
-      for (int t = 0; t < NOUT; ++t) dst[((int64_t)r * NOUT + t) * slab + opos] = out[t];
+      for (int t = 0; t < NOUT; ++t) {
+        *op = out[t];
+        op += slab;
+      }
       if constexpr (kind == 1) {
 #pragma unroll
         for (int sb = 0; sb < NB; ++sb) {
