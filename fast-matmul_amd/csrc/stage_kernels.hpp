@@ -36,13 +36,13 @@ struct FmVec {
 };
 
 template <typename T, int VEC>
-__device__ __forceinline__ FmVec<T, VEC> fm_load(const T* p) {
+__device__ __forceinline__ FmVec<T, VEC> fm_load(fm_gptr<const T> p) {
   FmVec<T, VEC> r;
   if constexpr (VEC == 1) {
     r.v[0] = *p;
   } else {
     typedef T vt __attribute__((ext_vector_type(VEC)));
-    vt x = *reinterpret_cast<const vt*>(p);
+    vt x = *(fm_gptr<const vt>)p;
 #pragma unroll
     for (int i = 0; i < VEC; ++i) r.v[i] = x[i];
   }
@@ -50,7 +50,7 @@ __device__ __forceinline__ FmVec<T, VEC> fm_load(const T* p) {
 }
 
 template <typename T, int VEC>
-__device__ __forceinline__ void fm_store(T* p, const FmVec<T, VEC>& r) {
+__device__ __forceinline__ void fm_store(fm_gptr<T> p, const FmVec<T, VEC>& r) {
   if constexpr (VEC == 1) {
     *p = r.v[0];
   } else {
@@ -58,7 +58,7 @@ __device__ __forceinline__ void fm_store(T* p, const FmVec<T, VEC>& r) {
     vt x;
 #pragma unroll
     for (int i = 0; i < VEC; ++i) x[i] = r.v[i];
-    *reinterpret_cast<vt*>(p) = x;
+    *(fm_gptr<vt>)p = x;
   }
 }
 
@@ -75,8 +75,8 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage(FmStageArgs a) {
   const int q = blockIdx.y;
   const FmNodeView nv = fm_node_view(a, q, NOUT, 0);
   const FmOp op = nv.op;
-  const T* src = reinterpret_cast<const T*>(fm_base(a.bases, op.role)) + op.off;
-  T* dst = reinterpret_cast<T*>(a.bases.p[FM_ROLE_WS]) + nv.out;
+  const fm_gptr<const T> src = fm_global(reinterpret_cast<const T*>(fm_base(a.bases, op.role)) + op.off);
+  const fm_gptr<T> dst = fm_global(reinterpret_cast<T*>(a.bases.p[FM_ROLE_WS]) + nv.out);
   const uint32_t rv = (uint32_t)nv.rows / VEC;
   const uint32_t total = rv * (uint32_t)nv.cols;
   const int64_t ld = op.ld, rows = nv.rows, cols = nv.cols, out_ld = nv.out_ld, slab = nv.out_slab;
@@ -120,8 +120,8 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage(FmStageArgs a) {
   const int q = blockIdx.y;
   const FmNodeView nv = fm_node_view(a, q, R, 0);
   const FmOp op = nv.op;
-  T* c = reinterpret_cast<T*>(fm_base(a.bases, op.role)) + op.off;
-  const T* mb = reinterpret_cast<const T*>(a.bases.p[FM_ROLE_WS]) + nv.out;
+  const fm_gptr<T> c = fm_global(reinterpret_cast<T*>(fm_base(a.bases, op.role)) + op.off);
+  const fm_gptr<const T> mb = fm_global(reinterpret_cast<const T*>(a.bases.p[FM_ROLE_WS]) + nv.out);
   const uint32_t rv = (uint32_t)nv.rows / VEC;
   const uint32_t total = rv * (uint32_t)nv.cols;
   const int64_t ld = op.ld, rows = nv.rows, cols = nv.cols, out_ld = nv.out_ld, slab = nv.out_slab;
@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage(FmStageArgs a) {
 #pragma unroll
     for (int t = 0; t < MN; ++t) {
       const int bi = t / Cmb::n, bj = t % Cmb::n;
-      T* p = c + (int64_t)bi * rows + i + ((int64_t)bj * cols + j) * ld;
+      const fm_gptr<T> p = c + (int64_t)bi * rows + i + ((int64_t)bj * cols + j) * ld;
       if constexpr (BETA) {
         FmVec<T, VEC> old = fm_load<T, VEC>(p);
 #pragma unroll
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage2(FmStageArgs a) {
   if ((int)blockIdx.x >= chunks * nv.cols2) return;  // grouped launch: a smaller node
   const int j = blockIdx.x / chunks;           // column within the sub-block
   const int i0 = (blockIdx.x - j * chunks) * P;  // first row
-  const T* mb = reinterpret_cast<const T*>(a.bases.p[FM_ROLE_WS]) + nv.out;
+  const fm_gptr<const T> mb = fm_global(reinterpret_cast<const T*>(a.bases.p[FM_ROLE_WS]) + nv.out);
   const int64_t col_off = (int64_t)j * nv.out_ld;
   const int64_t slab = nv.out_slab;
   const double x = nv.x;
@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage2(FmStageArgs a) {
     const int i = i0 + p;
     if (i >= rows2) continue;
     T mv[R];
-    const T* src = mb + (int64_t)r * R * slab + i + col_off;
+    const fm_gptr<const T> src = mb + (int64_t)r * R * slab + i + col_off;
 #pragma unroll
     for (int r2 = 0; r2 < R; ++r2) mv[r2] = src[(int64_t)r2 * slab];
     T out[MN];
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage2(FmStageArgs a) {
   __syncthreads();
   // 2. level-l W, one (row, sub-block t) item per thread and pass
   const FmOp op = nv.op;
-  T* c = reinterpret_cast<T*>(fm_base(a.bases, op.role)) + op.off;
+  const fm_gptr<T> c = fm_global(reinterpret_cast<T*>(fm_base(a.bases, op.role)) + op.off);
   const int64_t ld = op.ld;
   const T beta = (T)nv.beta;
   for (int item = threadIdx.x; item < P * MN; item += FM_STAGE_THREADS) {
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_w_stage2(FmStageArgs a) {
 #pragma unroll
     for (int b = 0; b < MN; ++b) {
       const int bi = b / Cmb::n, bj = b % Cmb::n;
-      T* dst = c + (int64_t)bi * nv.rows + r0 + ((int64_t)bj * nv.cols + c0) * ld;
+      const fm_gptr<T> dst = c + (int64_t)bi * nv.rows + r0 + ((int64_t)bj * nv.cols + c0) * ld;
       T v = out[b];
       if constexpr (BETA) v = fm_add(v, fm_mul(beta, *dst));
       *dst = v;
