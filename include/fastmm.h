@@ -86,7 +86,7 @@ fm_status fm_workspace_query(const char* alg, fm_dtype dtype, fm_mode mode, int 
  * par_util.hpp:17-95) and its leaf kernel by the linked BLAS.  Here every
  * choice a plan embeds is an explicit option, given per call or per plan:
  *   gemm_tile        -1: chosen per leaf launch by the tile cost model;
- *                    0 .. 31 forces an FmTileCfg (see fm_set_gemm_tile);
+ *                    0 .. 34 forces an FmTileCfg (see fm_set_gemm_tile);
  *   dfs_levels       top recursion levels run depth-first (0: breadth-first;
  *                    -1: as few as keep the workspace under workspace_limit,
  *                    or, with no limit, under a quarter of the device's
@@ -255,11 +255,12 @@ fm_status fm_probe_mfma_peak(int kind, int iters, double* tera_ops_per_s);
 fm_status fm_set_ozaki_slices(int slices);
 
 /* Process default of fm_options.gemm_tile: force leaf-GEMM tile configuration
- * `cfg` (0 .. 31, see FmTileCfg in the sources) for calls that pass no
+ * `cfg` (0 .. 34, see FmTileCfg in the sources) for calls that pass no
  * options; -1 restores the automatic choice.  Configurations 24 .. 29 and 31 run
  * split_bf16 leaves on prepared slice planes and are refused in other modes
  * (FM_ERR_INVALID_ARG when a plan is built); in ozaki_int8 mode, 1 (64 x 64)
- * selects the 16x16x64 digit kernel and any other value the default one. */
+ * selects the 16x16x64 digit kernel, 2 (128 x 64) the 32x32x32 one on 64 x 64
+ * tiles and any other value the default one (32x32x32 on 64 x 128 tiles). */
 fm_status fm_set_gemm_tile(int cfg);
 
 /* Diagonal scaling around the fast multiply (scaling/scaling.hpp:105-196,
