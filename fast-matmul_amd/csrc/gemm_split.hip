@@ -288,6 +288,7 @@ __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args) {
   // [k][row] of one slice (A sources); rows padded to 66 words so that the
   // reads below (16 rows of 4 k groups per wave, 8 k apart) hit 64 banks
   __shared__ float stage[32][64 + 2];
+  __shared__ fm_u32x4 ostage[64 * 12];  // the slice's planes, 64 rows x 192 bytes
   // A sources: thread t loads row t % 64 of columns t / 64 + 4 i
   const int lr = tid & 63, lk = tid >> 6;
   const bool lrok = r0 + lr < d.rows;
@@ -312,9 +313,16 @@ __global__ void __launch_bounds__(256) fm_split_prep(FmSplitArgs args) {
     }
     uint4 sl[3];
     fm_split3(x, sl);
-    const fm_gptr<char> dst = ws + d.dst + ((int64_t)kc * d.rows_pad + row) * 192 + q * 16;
+    // the slice's 64 rows x 192 bytes are one contiguous run of the
+    // planes: staged in LDS in that order, then written 16 bytes a lane
+    // over consecutive addresses (each store instruction a 1 KB run)
 #pragma unroll
-    for (int p = 0; p < 3; ++p) *(fm_gptr<fm_u32x4>)(dst + p * 64) = *reinterpret_cast<const fm_u32x4*>(&sl[p]);
+    for (int p = 0; p < 3; ++p) ostage[r * 12 + p * 4 + q] = *reinterpret_cast<const fm_u32x4*>(&sl[p]);
+    __syncthreads();
+    const fm_gptr<fm_u32x4> dst = (fm_gptr<fm_u32x4>)(ws + d.dst + ((int64_t)kc * d.rows_pad + r0) * 192);
+#pragma unroll
+    for (int u = 0; u < 3; ++u) dst[tid + 256 * u] = ostage[tid + 256 * u];
+    __syncthreads();  // the next slice overwrites the stage
   }
 }
 
