@@ -64,11 +64,13 @@ __device__ __forceinline__ int fm_xcd_remap(int wg, int nwg) {
 // interleaves the next step's LDS reads with the current step's MFMAs
 // (one read, then two MFMAs) through scheduling-group barriers; PIPE == 3
 // is PIPE == 2 with the next slice's LDS stores and the slice after's
-// global loads spread over the K steps of the current slice (full-tile
-// launches; others run as PIPE == 2).  FULL: every
-// product is a whole number of tiles and K slices with 16-byte aligned
-// columns (the launch checks), so only the unpredicated 16-byte load path
-// is compiled.
+// global loads spread over the K steps of the current slice (FULL and EVEN
+// launches; others run as PIPE == 2).  FULL: every product is a whole
+// number of tiles and K slices with 16-byte aligned columns (the launch
+// checks), so only the unpredicated 16-byte load path is compiled.  EVEN:
+// every product has whole K slices, aligned columns and M a multiple of
+// the vector width, so every tile, edge tiles included, takes 16-byte
+// loads.
 template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW = 1, int UNR = BK / 4,
           int PIPE = 0, bool FULL = false, bool EVEN = false>
 __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs args) {
@@ -93,19 +95,16 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   static_assert(A_PER * NT == BM * BK && B_PER * NT == BK * BN, "tile/thread mismatch");
   static_assert(TM >= 1 && TN >= 1 && BK % 4 == 0, "bad tile");
   typedef T vec_t __attribute__((ext_vector_type(VL)));
-  // Operand pointers: global address space for FP32 (global loads, which
-  // the LDS wait counter does not track), generic for FP64.  Measured on
-  // MI355X at 8192^3: FP32 tiles gain 1-4% with global loads; the FP64
-  // 128 x 128 two-groups-per-CU tile (the large-product choice) loses 4%
-  // (70.8 -> 67.7 TFLOP/s), other FP64 tiles move within +-2%.
-  // the spread loop runs in whole-tile (FULL) and whole-slice (EVEN: the
-  // launch guarantees K % BK == 0, aligned columns and M % VL == 0 for
-  // every product, so every tile takes 16-byte loads) launches; other
-  // PIPE == 3 launches run the PIPE == 2 loop
+  // the spread loop (PIPE == 3 on FULL and EVEN launches); other PIPE == 3
+  // launches run the PIPE == 2 loop
   constexpr bool SPREAD = PIPE == 3 && (FULL || EVEN);
   constexpr int PIPE_EFF = PIPE == 3 ? 2 : PIPE;
-  // (the spread loop waits on each part's global load separately: it needs
-  // the global address space whatever the element type)
+  // Operand pointers: global address space (global loads, which the LDS
+  // wait counter does not track) for FP32 and for the spread loop, which
+  // waits on each part's load separately; generic for the other FP64
+  // loops.  Measured on MI355X at 8192^3: FP32 tiles gain 1-4% with global
+  // loads; the FP64 PIPE == 2 128 x 128 two-groups-per-CU tile loses 4%
+  // (70.8 -> 67.7 TFLOP/s), other FP64 tiles move within +-2%.
   constexpr bool GM = sizeof(T) == 4 || SPREAD;
 
   __shared__ __align__(16) T As[2][BK][BM];
