@@ -68,9 +68,9 @@ __device__ __forceinline__ int fm_xcd_remap(int wg, int nwg) {
 // launches; others run as PIPE == 2).  FULL: every product is a whole
 // number of tiles and K slices with 16-byte aligned columns (the launch
 // checks), so only the unpredicated 16-byte load path is compiled.  EVEN:
-// every product has whole K slices, aligned columns and M a multiple of
-// the vector width, so every tile, edge tiles included, takes 16-byte
-// loads.
+// every product has M and K multiples of the vector width and aligned
+// columns, so every tile, edge tiles and a partial last K slice included,
+// takes 16-byte loads.
 template <typename T, int BM, int BN, int BK, int WM, int WN, int MINW = 1, int UNR = BK / 4,
           int PIPE = 0, bool FULL = false, bool EVEN = false>
 __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs args) {
@@ -265,14 +265,16 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
   // epilogue stores out of the next tile's main loop, where the compiler
   // would otherwise have to wait for every load and store in flight)
   bool vec_stage = FULL || EVEN || (interior && BK <= K);
-  if (vec_stage) {
-    gload_vec(0);
-    sstore_vec(0);
-  } else {
-    gload(0);
-    sstore(0);
+  if constexpr (!SPREAD) {  // the spread loop stages its first slice by parts
+    if (vec_stage) {
+      gload_vec(0);
+      sstore_vec(0);
+    } else {
+      gload(0);
+      sstore(0);
+    }
+    __syncthreads();
   }
-  __syncthreads();
   if constexpr (SPREAD) {
     // The PIPE == 2 loop, with the LDS stores of slice kt + 1 and the global
     // loads of slice kt + 2 cut in NS4 parts, one per K step of slice kt
@@ -295,28 +297,33 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
 #pragma unroll
         for (int j = 0; j < TN; ++j) acc[i][j] = Op::mma(bf[set][j], af[set][i], acc[i][j]);
     };
-    // Edge tiles: a vector outside the product loads from the product's
-    // first row (A) or column (B) instead, in bounds, so every lane loads
+    // Edge tiles and a partial last K slice (EVEN launches: K a multiple
+    // of the vector width, so a B vector's VL k lie wholly inside or
+    // outside): a vector outside the product loads from the product's first
+    // row, column or k instead, in bounds, so every lane loads
     // unconditionally (no branch for the compiler's wait counts to merge
     // over), and is zeroed when stored to LDS (zeroing it at the load would
     // make the compiler wait for the load right there).
     const bool a_ok = edge_free || av_in;
     const fm_ptr<GM, const T> av_src = a_ok ? av_ptr : A + (int64_t)av_k * lda;
     auto b_ok = [&](int i) { return edge_free || n0 + bv_col + i * BV_NSTEP < N; };
-    auto store_part = [&](int buf, int q) {
+    auto ak_ok = [&](int k0, int i) { return FULL || k0 + av_k + i * AV_KSTEP < K; };
+    auto bk_ok = [&](int k0) { return FULL || k0 + bv_k < K; };
+    // store part q of the slice at k0 (loaded by load_part(k0, q))
+    auto store_part = [&](int buf, int q, int k0) {
 #pragma unroll
       for (int i = 0; i < AV_PER; ++i)
         if (i % NS4 == q) {
           vec_t v;
 #pragma unroll
           for (int e = 0; e < VL; ++e) v[e] = ra[i * VL + e];
-          if constexpr (!FULL) v = a_ok ? v : vec_t{};
+          if constexpr (!FULL) v = a_ok && ak_ok(k0, i) ? v : vec_t{};
           *reinterpret_cast<vec_t*>(&As[buf][av_k + i * AV_KSTEP][av_row]) = v;
         }
 #pragma unroll
       for (int i = 0; i < BV_PER; ++i)
         if (i % NS4 == q) {
-          const bool ok = FULL || b_ok(i);
+          const bool ok = FULL || (b_ok(i) && bk_ok(k0));
 #pragma unroll
           for (int e = 0; e < VL; ++e)
             Bs[buf][bv_col + i * BV_NSTEP][bv_k + e] = ok ? rb[i * VL + e] : T(0);
@@ -326,7 +333,8 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
 #pragma unroll
       for (int i = 0; i < AV_PER; ++i)
         if (i % NS4 == q) {
-          const vec_t v = *(const fm_ptr<GM, const vec_t>)(av_src + (int64_t)(k0 + i * AV_KSTEP) * lda);
+          const int64_t kc = ak_ok(k0, i) ? (int64_t)(k0 + i * AV_KSTEP) : -(int64_t)av_k;
+          const vec_t v = *(const fm_ptr<GM, const vec_t>)(av_src + kc * lda);
 #pragma unroll
           for (int e = 0; e < VL; ++e) ra[i * VL + e] = v[e];
         }
@@ -334,11 +342,21 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
       for (int i = 0; i < BV_PER; ++i)
         if (i % NS4 == q) {
           const int64_t col = FULL || b_ok(i) ? (int64_t)(i * BV_NSTEP) : -(int64_t)(n0 + bv_col);
-          const vec_t v = *(const fm_ptr<GM, const vec_t>)(bv_ptr + k0 + col * ldb);
+          const int kr = bk_ok(k0) ? k0 : -bv_k;
+          const vec_t v = *(const fm_ptr<GM, const vec_t>)(bv_ptr + kr + col * ldb);
 #pragma unroll
           for (int e = 0; e < VL; ++e) rb[i * VL + e] = v[e];
         }
     };
+    // first slice
+#pragma unroll
+    for (int q = 0; q < NS4; ++q) {
+      load_part(0, q);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int q = 0; q < NS4; ++q) store_part(0, q, 0);
+    __syncthreads();
     // The stores and loads run unconditionally (the last slices store into
     // a buffer no one reads again and reload slice nk - 1, in bounds):
     // under a branch the compiler would wait for every load in flight
@@ -357,7 +375,7 @@ __global__ void __launch_bounds__(64 * WM * WN, MINW) fm_gemm_kernel(FmGemmArgs 
       const int kl = min(kt + 2, nk - 1) * BK;
 #pragma unroll
       for (int q = 0; q < NS4; ++q) {
-        store_part(cur ^ 1, q);
+        store_part(cur ^ 1, q, (kt + 1) * BK);
         load_part(kl, q);
         __builtin_amdgcn_sched_barrier(0);
         if (q < NS4 - 1) {

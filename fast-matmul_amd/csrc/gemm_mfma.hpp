@@ -27,8 +27,8 @@ struct FmGemmArgs {
                     // products whose own offsets and leading dimensions keep columns aligned
   int32_t grid;     // > 0: persistent launch with this many work groups
   int32_t full;     // every product is a whole number of tiles and K slices
-  int32_t even;     // every product has whole K slices, 16-byte aligned columns and
-                    // M a multiple of the vector width (edge tiles in M and N allowed)
+  int32_t even;     // every product has M and K multiples of the vector width and
+                    // 16-byte aligned columns (edge tiles and a partial last K slice)
 };
 
 // Tile configurations of the leaf kernel: BM x BN output tile, BK-deep K
@@ -74,8 +74,8 @@ enum FmTileCfg : int {
   // split-bf16 on slice planes, B fragments loaded straight into registers
   FM_TILE_SP_128x256_BD = 31,  // 2x4 waves, A through double-buffered LDS
   // as 22 and 23, with the next slice's LDS stores and global loads spread
-  // over the K steps (edge launches too, on products whose K is a multiple
-  // of 16 and whose columns are 16-byte aligned)
+  // over the K steps (edge launches too, on products whose M and K are
+  // multiples of the vector width and whose columns are 16-byte aligned)
   FM_TILE_PS_128x128_O2 = 32,
   FM_TILE_PS_128x256_W8 = 33,
   FM_TILE_PS_128x96_O2 = 34,   // as 30, spread

@@ -165,7 +165,7 @@ Launch gemm_launch(std::vector<FmGemmDesc> descs, int cfg, int stage, fm_dtype d
   for (const FmGemmDesc& d : descs) {
     vec = vec && d.a.off % vl == 0 && d.a.ld % vl == 0 && d.b.off % vl == 0 && d.b.ld % vl == 0;
     full = full && d.M % bm == 0 && d.N % bn == 0 && d.K % bk == 0 && d.K > 0;
-    even = even && d.M % vl == 0 && d.K % bk == 0 && d.K > 0;
+    even = even && d.M % vl == 0 && d.K % vl == 0 && d.K > 0;
   }
   L.vec_ok = 1;
   L.gm.full = (full && vec) ? 1 : 0;
@@ -419,12 +419,13 @@ bool whole_tiles(fm_dtype dt, int cfg, const std::vector<FmGemmDesc>& prods) {
   return true;
 }
 
-// Every product has whole K slices, 16-byte aligned columns and M a
-// multiple of the vector width (the spread kernels' EVEN launch).
+// Every product has M and K multiples of the vector width and 16-byte
+// aligned columns (the spread kernels' EVEN launch).
 bool even_slices(fm_dtype dt, int cfg, const std::vector<FmGemmDesc>& prods) {
-  const int bk = fm_tile_bk(cfg), vl = dt == FM_F64 ? 2 : 4;
+  (void)cfg;
+  const int vl = dt == FM_F64 ? 2 : 4;
   for (const FmGemmDesc& d : prods)
-    if (d.M % vl || d.K % bk || d.K == 0 || d.a.off % vl || d.a.ld % vl || d.b.off % vl ||
+    if (d.M % vl || d.K % vl || d.K == 0 || d.a.off % vl || d.a.ld % vl || d.b.off % vl ||
         d.b.ld % vl)
       return false;
   return true;
