@@ -89,6 +89,9 @@ fm_status fmplan::build_single(Plan& P, const FmAlgDesc* alg, fm_dtype dt, fm_mo
                                int64_t k, int64_t n, int64_t lda, int64_t ldb, int64_t ldc,
                                int steps, double x, double alpha, double beta,
                                const fm_options& o) {
+  // every launch numbers its work groups (tiles) in 32 bits
+  if (((m + 63) / 64) * ((n + 63) / 64) > 2147483647LL)
+    return set_err(FM_ERR_SHAPE, "product too large: more than 2^31 - 1 output tiles of 64 x 64");
   P.dtype = dt;
   P.mode = mode;
   P.slices = mode == FM_MODE_OZAKI_INT8 ? o.ozaki_slices : 0;

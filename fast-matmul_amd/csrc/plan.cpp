@@ -978,11 +978,13 @@ fm_status assemble(Plan& P, std::vector<SubPlan>& subs, int64_t ws_elems, int pe
       if (persistent_grid == 0 && P.split_rem)
         split_remainder(P, cfg_main, leaves, rest, &cfg_rest, &cfg_head);
       auto leaf = [&](std::vector<FmGemmDesc> prods, int c) {
-        int64_t kmax = 0;
-        for (const FmGemmDesc& d : prods) kmax = std::max<int64_t>(kmax, d.K);
-        // the preparation grid has one row of work groups per 32-deep K
-        // slice (at most 65535): deeper leaves split on the fly instead
-        if (fm_tile_planes(c) && kmax > 65535LL * 32) c = FM_TILE_128x128;
+        // the preparation launch is one flat grid of at most 2^31 - 1 work
+        // groups (64 rows x 4 K slices each, per operand; an upper bound
+        // here): beyond that the leaves split on the fly instead
+        int64_t wgs = 0;
+        for (const FmGemmDesc& d : prods)
+          wgs += ((int64_t)d.M + d.N + 126) / 64 * ((d.K + 127) / 128);
+        if (fm_tile_planes(c) && wgs > 2147483647LL) c = FM_TILE_128x128;
         if (fm_tile_planes(c)) return split_launch(std::move(prods), c, &wsb);
         return gemm_launch(std::move(prods), c, FM_STAGE_LEAF, P.dtype);
       };

@@ -119,12 +119,19 @@ def test_split_leaves_on_slice_planes():
     assert p["workspace_bytes"] == 384 * 128 * 6 + 256 * 128 * 6
 
 
-def test_split_leaves_deeper_than_the_prep_grid_split_on_the_fly():
-    # more than 65535 K slices of 32: the in-loop split kernel (FmTileCfg 0)
+def test_split_leaves_beyond_the_prep_grid_split_on_the_fly():
+    # the preparation is one flat launch: a very deep K still takes the
+    # slice planes; more than 2^31 - 1 preparation work groups (2^20 x 2^24
+    # x 2^20: 32767 row blocks x 131072 K blocks) take the in-loop split
+    # kernel (FmTileCfg 0)
     p = fastmm.plan_info("strassen", 64, 2200000, 64, 0, dtype=torch.float32, mode="split_bf16")
-    assert p["leaf"][0]["cfg"] == 0
-    p = fastmm.plan_info("strassen", 64, 2000000, 64, 0, dtype=torch.float32, mode="split_bf16")
     assert p["leaf"][0]["cfg"] >= 24
+    p = fastmm.plan_info("strassen", 1 << 20, 1 << 24, 1 << 20, 0, dtype=torch.float32,
+                         mode="split_bf16")
+    assert p["leaf"][0]["cfg"] == 0
+    # beyond 2^31 - 1 output tiles of 64 x 64 no launch can number its tiles
+    with pytest.raises(ValueError):
+        fastmm.plan_info("strassen", 1 << 24, 64, 1 << 24, 0, dtype=torch.float32)
 
 
 def test_stage_blocks_beyond_32_bit_counts_are_refused():
