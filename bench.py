@@ -149,6 +149,28 @@ def roofline_block(prof, plan, leaf_ops, leaf_peak, leaf_unit, leaf_kind):
     return out
 
 
+def stage_patterns(t, steps, M, K, N, stage_bytes):
+    """Slab counts of the two-level S, T and W stages (both levels in one
+    launch): (source slabs, output slabs, elements per slab).  S reads the
+    (m k)^2 sub-blocks of A and writes the rest of its bytes as slabs of the
+    same size; T likewise on B; W reads the R^2 products and writes C's
+    (m n)^2 sub-blocks."""
+    if steps != 2:
+        return {}
+    m, k, n = t.m, t.k, t.n
+    if M % (m * m) or K % (k * k) or N % (n * n):
+        return {}
+    sl = {"S": (M // (m * m)) * (K // (k * k)), "T": (K // (k * k)) * (N // (n * n)),
+          "W": (M // (m * m)) * (N // (n * n))}
+    reads = {"S": (m * k) ** 2, "T": (k * n) ** 2, "W": t.R ** 2}
+    out = {}
+    for key in ("S", "T", "W"):
+        total = stage_bytes[key] / (8.0 * sl[key])
+        if abs(total - round(total)) < 1e-6 and round(total) > reads[key]:
+            out[key] = (reads[key], int(round(total)) - reads[key], sl[key])
+    return out
+
+
 def measured_peaks(fastmm, torch, dev):
     """The matrix units' sustained rates (fm_probe_mfma_peak) and the HBM
     bandwidth of plain streaming kernels on this GPU."""
@@ -436,6 +458,17 @@ def main():
             st.update({"algorithmic_GB": plan_bytes[k] / 1e9, "GB_per_s": gbs,
                        "hbm_frac": gbs / PEAK_HBM_GBS})
         stages[k] = st
+    # each stage against the measured rate of its own access pattern
+    # (fm_probe_stream: one element position of every source slab read, of
+    # every output slab written, per lane)
+    for k, pat in stage_patterns(t_tab, steps, M, K, N, plan_bytes).items():
+        if stages.get(k, {}).get("GB_per_s"):
+            try:
+                pgbs = fastmm.probe_stream(*pat)
+                stages[k].update({"pattern": {"reads": pat[0], "writes": pat[1], "slab_elems": pat[2]},
+                                  "pattern_GBps": pgbs, "pattern_frac": stages[k]["GB_per_s"] / pgbs})
+            except Exception as e:  # reported, never fatal
+                stages[k]["pattern_GBps"] = "error: %s" % e
 
     if args.copy_probe:
         x = torch.empty(1 << 27, dtype=torch.float64, device=dev).fill_(1.0)

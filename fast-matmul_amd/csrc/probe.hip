@@ -1,9 +1,13 @@
-// probe.hip -- measured peaks of the matrix units on this GPU, for the
-// roofline figures: every SIMD issues back-to-back MFMAs on register
-// operands (no memory traffic) for a fixed count, timed with HIP events.
-// The rate this gives is the one the chip sustains at the clock it runs
-// under that load, which can sit below the published figure (quoted at the
-// peak engine clock).
+// probe.hip -- measured peaks on this GPU, for the roofline figures:
+// * the matrix units: every SIMD issues back-to-back MFMAs on register
+//   operands (no memory traffic) for a fixed count, timed with HIP events.
+//   The rate this gives is the one the chip sustains at the clock it runs
+//   under that load, which can sit below the published figure (quoted at
+//   the peak engine clock);
+// * HBM under a stage kernel's access pattern: each lane reads one element
+//   position of `reads` slabs and writes it to `writes` slabs, as the S, T
+//   and W stages do (with hundreds of output slabs the rate sits well below
+//   a single write stream's).
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -90,9 +94,85 @@ hipError_t launch_kind(int kind, int blocks, int iters, float* out, hipStream_t 
   }
 }
 
+// Stage-pattern stream: position e of `reads` input slabs and `writes`
+// output slabs (slab t at t * elems), one lane per position.  Sources are
+// loaded 64 at a time, all in flight together as in the stage kernels (a
+// last partial batch reloads the last slab, which the cache serves).
+constexpr int kProbeBatch = 64;
+
+__global__ void __launch_bounds__(256) fm_stream_probe(const double* __restrict__ in,
+                                                       double* __restrict__ out, int reads,
+                                                       int writes, int64_t elems) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < elems;
+       e += (int64_t)gridDim.x * 256) {
+    double acc = 0.0;
+    for (int b0 = 0; b0 < reads; b0 += kProbeBatch) {
+      double v[kProbeBatch];
+#pragma unroll
+      for (int i = 0; i < kProbeBatch; ++i)
+        v[i] = in[(int64_t)min(b0 + i, reads - 1) * elems + e];
+#pragma unroll
+      for (int i = 0; i < kProbeBatch; ++i) acc += b0 + i < reads ? v[i] : 0.0;
+    }
+    double* op = out + e;
+#pragma unroll 16
+    for (int t = 0; t < writes; ++t) {
+      *op = acc + t;
+      op += elems;
+    }
+  }
+}
+
 }  // namespace
 
 using namespace fmplan;
+
+extern "C" fm_status fm_probe_stream(int reads, int writes, int64_t elems, double* gb_per_s) {
+  clear_error();
+  if (!gb_per_s || reads < 0 || writes < 1 || reads > 4096 || writes > 4096 || elems < 1 ||
+      elems > (1LL << 30) || (double)(reads + writes) * elems * 8 > 32e9)
+    return set_err(FM_ERR_INVALID_ARG,
+                   "reads 0..4096, writes 1..4096, 1 <= elems <= 2^30, at most 32 GB in all");
+  *gb_per_s = 0.0;
+  double *in = nullptr, *out = nullptr;
+  hipError_t e = hipSuccess;
+  if (reads > 0) e = hipMalloc(&in, sizeof(double) * (size_t)reads * (size_t)elems);
+  if (e == hipSuccess) e = hipMalloc(&out, sizeof(double) * (size_t)writes * (size_t)elems);
+  if (e == hipSuccess && reads > 0)
+    e = hipMemset(in, 0, sizeof(double) * (size_t)reads * (size_t)elems);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  fm_status st = FM_OK;
+  if (e != hipSuccess) {
+    st = hip_err(e, "hipMalloc(stream probe)");
+  } else if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    st = set_err(FM_ERR_HIP, "hipEventCreate");
+  } else {
+    const int64_t wgs = (elems + 255) / 256;
+    const dim3 grid((unsigned)(wgs < 65536 ? wgs : 65536));
+    const int reps = 5;
+    hipLaunchKernelGGL(fm_stream_probe, grid, dim3(256), 0, nullptr, in, out, reads, writes, elems);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+      hipLaunchKernelGGL(fm_stream_probe, grid, dim3(256), 0, nullptr, in, out, reads, writes,
+                         elems);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess)
+      st = hip_err(e, "stream probe launch");
+    else if (ms > 0.f)
+      *gb_per_s = (double)(reads + writes) * elems * 8 * reps / (ms * 1e-3) / 1e9;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (in) (void)hipFree(in);
+  if (out) (void)hipFree(out);
+  return st;
+}
 
 extern "C" fm_status fm_probe_mfma_peak(int kind, int iters, double* tera_ops_per_s) {
   // iters counts single MFMA rounds; the kernel runs them kRounds per trip

@@ -332,6 +332,15 @@ def probe_mfma_peak(kind: str, seconds: float = 0.05) -> float:
     return out.value
 
 
+def probe_stream(reads: int, writes: int, elems: int) -> float:
+    """Measured HBM rate (GB/s) of a stage kernel's access pattern: each lane
+    reads one element position of ``reads`` FP64 slabs of ``elems`` elements
+    and writes it to ``writes`` slabs (fm_probe_stream)."""
+    out = ctypes.c_double(0.0)
+    check(LIB.fm_probe_stream(int(reads), int(writes), int(elems), ctypes.byref(out)))
+    return out.value
+
+
 def profile_read():
     """{stage: (ms_total, launches)} accumulated since the last read."""
     p = _lib.FmProfile()

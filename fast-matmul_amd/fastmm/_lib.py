@@ -35,6 +35,7 @@ EXPORTS = (
     "fm_options_default", "fm_fastmatmul_ex", "fm_plan_create", "fm_plan_run",
     "fm_plan_workspace", "fm_plan_destroy", "fm_scaled_workspace_query", "fm_release_workspace",
     "fm_fastmatmul_noalloc", "fm_probe_mfma_peak", "fm_autotune_load", "fm_autotune_table",
+    "fm_probe_stream",
 )
 
 # the fixed scratch array of fm_fastmatmul_noalloc (FM_NOALLOC_SCRATCH_BYTES)
@@ -105,6 +106,8 @@ def _load():
     lib.fm_autotune_table.argtypes = []
     lib.fm_probe_mfma_peak.restype = c_int
     lib.fm_probe_mfma_peak.argtypes = [c_int, c_int, ctypes.POINTER(c_double)]
+    lib.fm_probe_stream.restype = c_int
+    lib.fm_probe_stream.argtypes = [c_int, c_int, i64, ctypes.POINTER(c_double)]
     lib.fm_options_default.restype = None
     lib.fm_options_default.argtypes = [ctypes.POINTER(FmOptions)]
     lib.fm_plan_create.restype = c_int

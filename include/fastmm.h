@@ -248,6 +248,15 @@ fm_status fm_profile_read(fm_profile* out);
  * 2: BF16 (16x16x32_bf16, FP32 accumulate), 3: INT8 (32x32x32_i8). */
 fm_status fm_probe_mfma_peak(int kind, int iters, double* tera_ops_per_s);
 
+/* Measured HBM rate of a stage kernel's access pattern on the current
+ * device, for roofline figures (no reference counterpart): each lane reads
+ * one element position of `reads` FP64 slabs of `elems` elements and writes
+ * it to `writes` such slabs; the result is (reads + writes) * elems * 8
+ * bytes over the kernel's time, in GB/s.  The S, T and W stages read and
+ * write whole slabs this way.  Allocates and frees its buffers (at most
+ * 32 GB). */
+fm_status fm_probe_stream(int reads, int writes, int64_t elems, double* gb_per_s);
+
 /* Process default of fm_options.ozaki_slices.
  * Digit count S of FM_MODE_OZAKI_INT8 leaves (6, 7 or 8; default 8): each
  * FP64 operand row/column becomes S int8 digits of 7 bits under a

@@ -55,6 +55,11 @@ def test_argument_errors():
     st = L.fm_fastmatmul(b"strassen", 0, 0, -1, 8, 8, nul, 8, nul, 8, nul, 8, 1, 1e-8, 1.0, 0.0,
                          None, None, 0, None)
     assert st == _lib.FM_ERR_INVALID_ARG
+    # probe sizes out of range fail before any device call
+    out = ctypes.c_double(0.0)
+    assert L.fm_probe_stream(4, 0, 1 << 20, ctypes.byref(out)) == _lib.FM_ERR_INVALID_ARG
+    assert L.fm_probe_stream(4096, 4096, 1 << 30, ctypes.byref(out)) == _lib.FM_ERR_INVALID_ARG
+    assert L.fm_probe_stream(4, 16, 0, ctypes.byref(out)) == _lib.FM_ERR_INVALID_ARG
     # empty output is a no-op
     st = L.fm_fastmatmul(b"strassen", 0, 0, 0, 8, 8, nul, 1, nul, 8, nul, 1, 1, 1e-8, 1.0, 0.0,
                          None, None, 0, None)
