@@ -17,6 +17,9 @@
 enum FmLeafPhase : int { FM_PHASE_PREP = 1, FM_PHASE_PRODUCT = 2, FM_PHASE_ALL = 3 };
 
 enum FmRole : int32_t { FM_ROLE_A = 0, FM_ROLE_B = 1, FM_ROLE_C = 2, FM_ROLE_WS = 3 };
+// plan-time marker of a leaf operand whose split-bf16 slice planes a stage
+// writes (FmOp::off: their byte offset in the workspace); never a kernel's role
+constexpr int32_t FM_ROLE_PLANES = -2;
 
 // Roles 0-3 come from p[]; a batched plan numbers its problems' operands
 // from 4 on and resolves them through a device table (ext[role - 4]).
@@ -148,6 +151,9 @@ struct FmAlgDesc {
   const int* b_view_slot;
   // two-level W stage (nullptr where a level's products do not fit in LDS)
   FmStageLaunch w2_d, w2_f;
+  // two-level S/T stages of FP32 data writing the leaves' split-bf16 slice
+  // planes (nullptr where s2_f / t2_f are)
+  FmStageLaunch s2p_f, t2p_f;
 };
 
 // generated registry (csrc/gen/registry.cpp): one accessor per algorithm

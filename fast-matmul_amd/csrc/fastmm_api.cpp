@@ -53,7 +53,7 @@ std::string plan_key(int dev, const char* alg, int dt, int mode, int64_t m, int6
   char buf[512];
   snprintf(buf, sizeof(buf), "%d|%s|%d|%d|%d|%d|%d|%d|%zu|%d|%lld|%lld|%lld|%lld|%lld|%lld|%d|%a|%a|%a",
            dev, alg ? alg : "<gemm>", dt, mode, mode == FM_MODE_OZAKI_INT8 ? o.ozaki_slices : 0,
-           o.stage_fusion != 0, o.split_remainder != 0, o.dfs_levels, o.workspace_limit, o.gemm_tile,
+           o.stage_fusion, o.split_remainder != 0, o.dfs_levels, o.workspace_limit, o.gemm_tile,
            (long long)m, (long long)k, (long long)n, (long long)lda, (long long)ldb,
            (long long)ldc, steps, x, alpha, beta);
   return buf;
@@ -110,6 +110,8 @@ fm_status fmplan::build_single(Plan& P, const FmAlgDesc* alg, fm_dtype dt, fm_mo
   p.alpha = alpha;
   p.beta = beta;
   p.fuse = o.stage_fusion != 0;
+  p.fuse_planes = o.stage_fusion == 1 && mode == FM_MODE_SPLIT_BF16 &&
+                  (P.tile < 0 || fm_tile_planes(P.tile));
   p.mode = mode;
   p.slices = P.slices;
   p.tile = P.tile;
@@ -283,7 +285,8 @@ double launch_bytes(const Launch& L, double es) {
       int mat = 0;
       for (int r = 0; r < g->R; ++r) mat += a ? g->a_kind[r] : g->b_kind[r];
       const double outs = (double)g->R * (a ? g->ns : g->nt) + (double)mat * (a ? g->a_nviews : g->b_nviews);
-      return es * nodes * L.st.rows2 * L.st.cols2 * ((double)nb * nb + outs);
+      // outputs as split-bf16 slice planes: 6 bytes an element
+      return (double)nodes * L.st.rows2 * L.st.cols2 * (es * nb * nb + (L.planes ? 6.0 : es) * outs);
     }
     case LK_W:
       return es * nodes * L.st.rows * L.st.cols *
@@ -722,7 +725,7 @@ fm_status fm_set_schedule(int dfs_levels, size_t workspace_limit, int split_rema
 
 void fm_set_stage_fusion(int on) {
   std::lock_guard<std::mutex> lock(fm_api_mutex());
-  g_defaults.stage_fusion = on != 0;
+  g_defaults.stage_fusion = on == 2 ? 2 : on != 0;
 }
 
 void fm_profile_enable(int on) {

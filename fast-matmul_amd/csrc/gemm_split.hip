@@ -19,49 +19,19 @@
 
 #include "gemm_mfma.hpp"
 #include "gmem.hpp"
+#include "split_bf16.hpp"
 
 typedef __bf16 fm_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float fm_f32x4 __attribute__((ext_vector_type(4)));
 // native 16-byte vector (register-resident in arrays, unlike the uint4 struct)
 typedef uint32_t fm_u32x4 __attribute__((ext_vector_type(4)));
 
-// round-to-nearest-even float -> bf16 (finite inputs; fm_split3 handles the rest)
-__device__ __forceinline__ uint16_t fm_bf16_bits(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float fm_bf16_to_f32(uint16_t b) {
-  return __uint_as_float((uint32_t)b << 16);
-}
-
-// split 8 values into three slice planes (8 bf16 each, packed in 4 uints).
-// Finite values whose rounding to bf16 would overflow (|x| at or above the
-// largest bf16 plus half an ulp) take the truncated value as the leading
-// slice instead, so their residuals stay finite (x - x0 is then exact: both
-// lie in the top binade).  Inf and NaN go through the leading slice with zero
-// residuals (a NaN keeps a set mantissa bit).
+// split 8 values into three slice planes (8 bf16 each, packed in 4 uints);
+// fm_split1 (split_bf16.hpp) per value
 __device__ __forceinline__ void fm_split3(const float (&x)[8], uint4 (&out)[3]) {
   uint32_t h[8], m[8], l[8];
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    const uint32_t u = __float_as_uint(x[e]);
-    uint16_t b0, b1, b2;
-    if ((u & 0x7F800000u) == 0x7F800000u) {
-      b0 = (uint16_t)((u >> 16) | ((u & 0x007FFFFFu) ? 0x0040u : 0u));
-      b1 = b2 = 0;
-    } else {
-      b0 = fm_bf16_bits(x[e]);
-      if ((b0 & 0x7F80u) == 0x7F80u) b0 = (uint16_t)(u >> 16);  // rounded to Inf: truncate
-      const float r1 = __fsub_rn(x[e], fm_bf16_to_f32(b0));
-      b1 = fm_bf16_bits(r1);
-      const float r2 = __fsub_rn(r1, fm_bf16_to_f32(b1));
-      b2 = fm_bf16_bits(r2);
-    }
-    h[e] = b0;
-    m[e] = b1;
-    l[e] = b2;
-  }
+  for (int e = 0; e < 8; ++e) fm_split1(x[e], h[e], m[e], l[e]);
   out[0] = make_uint4(h[0] | h[1] << 16, h[2] | h[3] << 16, h[4] | h[5] << 16, h[6] | h[7] << 16);
   out[1] = make_uint4(m[0] | m[1] << 16, m[2] | m[3] << 16, m[4] | m[5] << 16, m[6] | m[7] << 16);
   out[2] = make_uint4(l[0] | l[1] << 16, l[2] | l[3] << 16, l[4] | l[5] << 16, l[6] | l[7] << 16);

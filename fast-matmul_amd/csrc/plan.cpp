@@ -245,6 +245,7 @@ Launch split_launch(std::vector<FmGemmDesc> prods, int cfg, size_t* ws_bytes) {
   std::map<std::tuple<int, int64_t, int64_t, int, int, int>, int64_t> seen;
   int32_t prep_wgs = 0;
   auto planes = [&](const FmOp& src, int rows, int K, int pad_to, int kcontig) {
+    if (src.role == FM_ROLE_PLANES) return src.off;  // written by the S/T stage
     const auto key = std::make_tuple(src.role, src.off, src.ld, rows, K, kcontig);
     auto it = seen.find(key);
     if (it != seen.end()) return it->second;
@@ -304,6 +305,8 @@ fm_options g_defaults = {-1, 0, 0, 1, 1, 8};
 fm_status check_options(const fm_options& o) {
   if (o.gemm_tile < -1 || o.gemm_tile >= FM_TILE_NUM) return set_err(FM_ERR_INVALID_ARG, "unknown tile config");
   if (o.dfs_levels < -1) return set_err(FM_ERR_INVALID_ARG, "dfs_levels must be >= -1");
+  if (o.stage_fusion < 0 || o.stage_fusion > 2)
+    return set_err(FM_ERR_INVALID_ARG, "stage_fusion must be 0, 1 or 2");
   if (o.ozaki_slices < 6 || o.ozaki_slices > 8)
     return set_err(FM_ERR_INVALID_ARG, "ozaki_slices must be 6, 7 or 8");
   return FM_OK;
@@ -631,6 +634,7 @@ fm_status build_dfs(fm_dtype dt, const Problem& p, const std::vector<Dims>& dims
     Problem q = p;
     q.steps = p.steps - 1;
     q.dfs = p.dfs - 1;
+    q.fuse_planes = false;
     q.m = c.m;
     q.k = c.k;
     q.n = c.n;
@@ -719,44 +723,69 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
       }
     }
   }
+  // Split-bf16 leaves on slice planes: the two-level S (T) stage of the last
+  // two levels writes the leaf operands' planes itself, and the preparation
+  // pass covers only leaf operands that are sub-blocks of A (B).  Taken when
+  // every split kernel's tiles divide the leaves (rows multiples of 256) and
+  // K is whole 32-deep slices, so the planes need no padding, and when the
+  // leaves' preparation grid would fit one launch (else they split on the
+  // fly, on the FP32 operands).
+  bool s_pl = false, t_pl = false;
+  if (p.fuse_planes && dt == FM_F32 && L >= 2) {
+    const Dims d = dims[L];
+    double leaves = 1.0;
+    for (int l = 0; l < L; ++l) leaves *= R;
+    const double wgs = leaves * (double)((d.m + d.n + 126) / 64) * (double)((d.k + 127) / 128);
+    const bool ok = d.m % 256 == 0 && d.n % 256 == 0 && d.k % 32 == 0 && wgs <= 2147483647.0;
+    s_pl = ok && s2[L - 2] && alg->s2p_f != nullptr;
+    t_pl = ok && t2[L - 2] && alg->t2p_f != nullptr;
+  }
   const FmOp virt = op_make(-1, 0, 0);
   for (int l = 0; l < L; ++l) {
     const Dims c = dims[l + 1];
     const int64_t Q = (int64_t)lv[l].size();
     const int64_t s_ld = align_up(c.m, kLdAlign), s_slab = align_up(s_ld * c.k, kSlabAlign);
     const int64_t t_ld = align_up(c.k, kLdAlign), t_slab = align_up(t_ld * c.n, kSlabAlign);
+    // the last level's S (T) outputs and views as slice planes: bytes per
+    // operand (c.m x c.k, 6 bytes an element), in FP32 workspace elements
+    const bool spl = s_pl && l == L - 1, tpl = t_pl && l == L - 1;
+    const int64_t s_pb = align_up(c.m * c.k * 6, 256), t_pb = align_up(c.k * c.n * 6, 256);
+    const int64_t s_step = spl ? s_pb / 4 : s_slab, t_step = tpl ? t_pb / 4 : t_slab;
     m_ld[l] = align_up(c.m, kLdAlign);
     m_slab[l] = align_up(m_ld[l] * c.n, kSlabAlign);
     const bool s_first = s2[l], s_second = l > 0 && s2[l - 1];
     const bool t_first = t2[l], t_second = l > 0 && t2[l - 1];
     const int64_t s_off = ws;
-    if (!s_first) ws += Q * NS * s_slab;
+    if (!s_first) ws += Q * NS * s_step;
     const int64_t t_off = ws;
-    if (!t_first) ws += Q * NT * t_slab;
+    if (!t_first) ws += Q * NT * t_step;
     const int64_t sv_off = ws;
-    if (s_second) ws += Q * alg->a_nviews * s_slab;
+    if (s_second) ws += Q * alg->a_nviews * s_step;
     const int64_t tv_off = ws;
-    if (t_second) ws += Q * alg->b_nviews * t_slab;
+    if (t_second) ws += Q * alg->b_nviews * t_step;
     m_off[l] = ws;
     if (!w2[l]) ws += Q * R * m_slab[l];
     lv[l + 1].reserve(Q * R);
     for (int64_t q = 0; q < Q; ++q) {
       Node& nd = lv[l][q];
-      if (s_second && nd.a.role < 0) nd.a_views = sv_off + q * alg->a_nviews * s_slab;
-      if (t_second && nd.b.role < 0) nd.b_views = tv_off + q * alg->b_nviews * t_slab;
+      if (s_second && nd.a.role < 0) nd.a_views = sv_off + q * alg->a_nviews * s_step;
+      if (t_second && nd.b.role < 0) nd.b_views = tv_off + q * alg->b_nviews * t_step;
+      // a workspace operand, or (last level, planes) the byte offset of its planes
+      auto s_op = [&](int64_t off) { return spl ? op_make(FM_ROLE_PLANES, off * 4, 0) : op_make(FM_ROLE_WS, off, s_ld); };
+      auto t_op = [&](int64_t off) { return tpl ? op_make(FM_ROLE_PLANES, off * 4, 0) : op_make(FM_ROLE_WS, off, t_ld); };
       for (int r = 0; r < R; ++r) {
         Node ch;
         const int ai = alg->a_index[r], bi = alg->b_index[r];
         if (alg->a_kind[r])
-          ch.a = s_first ? virt : op_make(FM_ROLE_WS, s_off + (q * NS + ai) * s_slab, s_ld);
+          ch.a = s_first ? virt : s_op(s_off + (q * NS + ai) * s_step);
         else if (nd.a_views >= 0)
-          ch.a = op_make(FM_ROLE_WS, nd.a_views + alg->a_view_slot[ai] * s_slab, s_ld);
+          ch.a = s_op(nd.a_views + alg->a_view_slot[ai] * s_step);
         else
           ch.a = op_sub(nd.a, (ai / alg->k) * c.m, (ai % alg->k) * c.k);
         if (alg->b_kind[r])
-          ch.b = t_first ? virt : op_make(FM_ROLE_WS, t_off + (q * NT + bi) * t_slab, t_ld);
+          ch.b = t_first ? virt : t_op(t_off + (q * NT + bi) * t_step);
         else if (nd.b_views >= 0)
-          ch.b = op_make(FM_ROLE_WS, nd.b_views + alg->b_view_slot[bi] * t_slab, t_ld);
+          ch.b = t_op(nd.b_views + alg->b_view_slot[bi] * t_step);
         else
           ch.b = op_sub(nd.b, (bi / alg->n) * c.k, (bi % alg->n) * c.n);
         ch.c = w2[l] ? virt : op_make(FM_ROLE_WS, m_off[l] + (q * R + r) * m_slab[l], m_ld[l]);
@@ -769,8 +798,15 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
       const Dims pc = dims[l];
       std::vector<FmOp> ops;
       for (const Node& nd : lv[l - 1]) ops.push_back(nd.a);
-      out.pre.push_back(stage2_launch(LK_S2, alg, std::move(ops), pc.m, pc.k, c.m, c.k, s_off,
-                                      s_ld, s_slab, sv_off, p.x));
+      if (spl) {
+        Launch S = stage2_launch(LK_S2, alg, std::move(ops), pc.m, pc.k, c.m, c.k, s_off * 4, 0,
+                                 s_pb, sv_off * 4, p.x);
+        S.planes = 1;
+        out.pre.push_back(std::move(S));
+      } else {
+        out.pre.push_back(stage2_launch(LK_S2, alg, std::move(ops), pc.m, pc.k, c.m, c.k, s_off,
+                                        s_ld, s_slab, sv_off, p.x));
+      }
     } else if (NS > 0 && !s_first) {
       std::vector<FmOp> ops;
       for (const Node& nd : lv[l]) ops.push_back(nd.a);
@@ -781,8 +817,15 @@ fm_status build_subplan(fm_dtype dt, const Problem& p, int64_t ws_base, SubPlan&
       const Dims pc = dims[l];
       std::vector<FmOp> ops;
       for (const Node& nd : lv[l - 1]) ops.push_back(nd.b);
-      out.pre.push_back(stage2_launch(LK_T2, alg, std::move(ops), pc.k, pc.n, c.k, c.n, t_off,
-                                      t_ld, t_slab, tv_off, p.x));
+      if (tpl) {
+        Launch T = stage2_launch(LK_T2, alg, std::move(ops), pc.k, pc.n, c.k, c.n, t_off * 4, 0,
+                                 t_pb, tv_off * 4, p.x);
+        T.planes = 1;
+        out.pre.push_back(std::move(T));
+      } else {
+        out.pre.push_back(stage2_launch(LK_T2, alg, std::move(ops), pc.k, pc.n, c.k, c.n, t_off,
+                                        t_ld, t_slab, tv_off, p.x));
+      }
     } else if (NT > 0 && !t_first) {
       std::vector<FmOp> ops;
       for (const Node& nd : lv[l]) ops.push_back(nd.b);
@@ -1197,8 +1240,8 @@ fm_status Plan::run(const void* A, const void* B, void* C, void* ws, hipStream_t
       switch (L.kind) {
         case LK_S: fn = d ? g->s_d : g->s_f; break;
         case LK_T: fn = d ? g->t_d : g->t_f; break;
-        case LK_S2: fn = d ? g->s2_d : g->s2_f; break;
-        case LK_T2: fn = d ? g->t2_d : g->t2_f; break;
+        case LK_S2: fn = d ? g->s2_d : L.planes ? g->s2p_f : g->s2_f; break;
+        case LK_T2: fn = d ? g->t2_d : L.planes ? g->t2p_f : g->t2_f; break;
         case LK_W2: fn = d ? g->w2_d : g->w2_f; break;
         default: fn = d ? g->w_d : g->w_f; break;
       }

@@ -33,6 +33,7 @@ struct Launch {
   FmSplitArgs sp{};
   int cfg = 0;
   int vec_ok = 1;
+  int planes = 0;  // LK_S2 / LK_T2 writing split-bf16 slice planes (byte offsets in st)
   int oz_slices = 0, oz_rows = 0, oz_cols = 0;
   std::vector<FmOp> ops;
   std::vector<FmStageNode> snodes;  // grouped stage launch: per-node descriptors
@@ -74,6 +75,9 @@ struct Problem {
   double x = 1e-8, alpha = 1.0, beta = 0.0;
   int64_t off_a = 0, off_b = 0, off_c = 0;  // element offsets within the roles
   bool fuse = true;  // two-level S/T stages where the shape allows them
+  // split_bf16 leaves on slice planes: the last two-level S/T stage writes
+  // the leaf operands' planes itself (FP32 data; single problems only)
+  bool fuse_planes = false;
   int dfs = 0;       // top levels run depth-first (one child sub-plan at a time)
   fm_mode mode = FM_MODE_PLAIN;  // leaf mode and digit count, for depth-first children
   int slices = 0;

@@ -20,6 +20,7 @@
 
 #include "alg_desc.hpp"
 #include "gmem.hpp"
+#include "split_bf16.hpp"
 
 __device__ __forceinline__ double fm_add(double a, double b) { return __dadd_rn(a, b); }
 __device__ __forceinline__ double fm_sub(double a, double b) { return __dadd_rn(a, -b); }
@@ -262,6 +263,139 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2(FmStageArgs a) 
   }
 }
 
+// Two-level S (SIDE 0) or T (SIDE 1) of FP32 data written straight as the
+// leaves' split-bf16 slice planes (layout in split_bf16.hpp), for leaf
+// launches on slice planes: the values of fm_in_stage2 (same expressions,
+// same order), each split by fm_split1 as the preparation pass splits a
+// stored FP32 slab, so the planes are bit-identical to preparing
+// fm_in_stage2's output.  Here out_off, view_off and out_slab are bytes (one
+// slab's planes: rows2 * cols2 * 6 bytes, no padding: the plan takes this
+// path only when the leaf operands' rows are whole tiles and their K whole
+// 32-deep slices).  A lane owns VEC consecutive k of one plane row (A side:
+// row i, columns k; B side: column j, rows k), so 32 / VEC consecutive lanes
+// write one 64-byte run of each plane.
+template <class Cmb, int SIDE>
+constexpr int fm_planes_vec() {
+  constexpr int NB = SIDE == 0 ? Cmb::m * Cmb::k : Cmb::k * Cmb::n;
+  return NB * NB * 4 <= 96 ? 4 : NB * NB * 2 <= 96 ? 2 : 1;
+}
+
+template <int VEC>
+__device__ __forceinline__ void fm_put_planes(fm_gptr<char> p, const float (&val)[VEC]) {
+  uint32_t h[VEC], m[VEC], l[VEC];
+#pragma unroll
+  for (int e = 0; e < VEC; ++e) fm_split1(val[e], h[e], m[e], l[e]);
+  if constexpr (VEC == 4) {
+    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+    *(fm_gptr<u2>)p = u2{h[0] | h[1] << 16, h[2] | h[3] << 16};
+    *(fm_gptr<u2>)(p + 64) = u2{m[0] | m[1] << 16, m[2] | m[3] << 16};
+    *(fm_gptr<u2>)(p + 128) = u2{l[0] | l[1] << 16, l[2] | l[3] << 16};
+  } else if constexpr (VEC == 2) {
+    *(fm_gptr<uint32_t>)p = h[0] | h[1] << 16;
+    *(fm_gptr<uint32_t>)(p + 64) = m[0] | m[1] << 16;
+    *(fm_gptr<uint32_t>)(p + 128) = l[0] | l[1] << 16;
+  } else {
+    *(fm_gptr<uint16_t>)p = (uint16_t)h[0];
+    *(fm_gptr<uint16_t>)(p + 64) = (uint16_t)m[0];
+    *(fm_gptr<uint16_t>)(p + 128) = (uint16_t)l[0];
+  }
+}
+
+template <class Cmb, int SIDE>
+__global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2_planes(FmStageArgs a) {
+  using T = float;
+  constexpr int BC = SIDE == 0 ? Cmb::k : Cmb::n;
+  constexpr int NB = (SIDE == 0 ? Cmb::m : Cmb::k) * BC;
+  constexpr int R = Cmb::R;
+  constexpr int NOUT = SIDE == 0 ? Cmb::NS : Cmb::NT;
+  constexpr int NOUT_ARR = NOUT > 0 ? NOUT : 1;
+  constexpr int NV = SIDE == 0 ? Cmb::a_nviews : Cmb::b_nviews;
+  constexpr int VEC = fm_planes_vec<Cmb, SIDE>();
+  constexpr int KQ = 32 / VEC;  // lanes per 32-deep run of a plane row
+  const int q = blockIdx.y;
+  const FmNodeView nv = fm_node_view(a, q, R * NOUT, R * NV);
+  const FmOp op = nv.op;
+  const fm_gptr<const T> src = fm_global(reinterpret_cast<const T*>(fm_base(a.bases, op.role)) + op.off);
+  const fm_gptr<char> ws = fm_global(reinterpret_cast<char*>(a.bases.p[FM_ROLE_WS]));
+  const fm_gptr<char> dst = ws + nv.out;
+  const fm_gptr<char> vdst = ws + nv.view;
+  const uint32_t prow = SIDE == 0 ? (uint32_t)nv.rows2 : (uint32_t)nv.cols2;  // plane rows
+  const uint32_t nkc = (SIDE == 0 ? (uint32_t)nv.cols2 : (uint32_t)nv.rows2) / 32;
+  const uint32_t total = prow * nkc * KQ;
+  const int64_t ld = op.ld, brows = nv.rows, bcols = nv.cols, brows2 = nv.rows2,
+                bcols2 = nv.cols2, slab = nv.out_slab;
+  // the step between a lane's VEC elements: columns (A side) or rows (B side)
+  const int64_t estep = SIDE == 0 ? ld : 1;
+  const double x = nv.x;
+  for (uint32_t e = blockIdx.x * FM_STAGE_THREADS + threadIdx.x; e < total;
+       e += gridDim.x * FM_STAGE_THREADS) {
+    const uint32_t kq = e % KQ;
+    const uint32_t rest = e / KQ;
+    const uint32_t r = rest % prow;
+    const uint32_t kc = rest / prow;
+    const uint32_t k0 = kc * 32 + kq * VEC;
+    const uint32_t i = SIDE == 0 ? r : k0, j = SIDE == 0 ? k0 : r;
+    T v[NB][NB][VEC];  // v[block][sub-block][element]
+    const fm_gptr<const T> lsrc = src + i + (int64_t)j * ld;
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const fm_gptr<const T> bsrc = lsrc + (int64_t)(b / BC) * brows + (int64_t)(b % BC) * bcols * ld;
+#pragma unroll
+      for (int sb = 0; sb < NB; ++sb) {
+        const fm_gptr<const T> p = bsrc + (int64_t)(sb / BC) * brows2 + (int64_t)(sb % BC) * bcols2 * ld;
+#pragma unroll
+        for (int l = 0; l < VEC; ++l) v[b][sb][l] = p[l * estep];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    // the lane's run within one slab's planes
+    const int64_t pos = ((int64_t)kc * prow + r) * 192 + kq * (VEC * 2);
+    auto product = [&](auto rc) {
+      constexpr int rr = decltype(rc)::value;
+      constexpr int kind = SIDE == 0 ? Cmb::a_kind_c[rr] : Cmb::b_kind_c[rr];
+      constexpr int idx = SIDE == 0 ? Cmb::a_index_c[rr] : Cmb::b_index_c[rr];
+      T par[NB][VEC];
+      T out[NOUT_ARR][VEC];
+#pragma unroll
+      for (int l = 0; l < VEC; ++l) {
+        T pl[NB];
+#pragma unroll
+        for (int sb = 0; sb < NB; ++sb) {
+          if constexpr (kind == 0) {
+            pl[sb] = v[idx][sb][l];
+          } else {
+            T in[NB];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) in[b] = v[b][sb][l];
+            if constexpr (SIDE == 0)
+              pl[sb] = Cmb::template s_one<idx, T>(in, x);
+            else
+              pl[sb] = Cmb::template t_one<idx, T>(in, x);
+          }
+          par[sb][l] = pl[sb];
+        }
+        T o[NOUT_ARR];
+        if constexpr (SIDE == 0)
+          Cmb::s_combine(pl, o, x);
+        else
+          Cmb::t_combine(pl, o, x);
+#pragma unroll
+        for (int t = 0; t < NOUT; ++t) out[t][l] = o[t];
+      }
+#pragma unroll
+      for (int t = 0; t < NOUT; ++t) fm_put_planes<VEC>(dst + ((int64_t)rr * NOUT + t) * slab + pos, out[t]);
+      if constexpr (kind == 1) {
+#pragma unroll
+        for (int sb = 0; sb < NB; ++sb) {
+          const int slot = SIDE == 0 ? Cmb::a_view_slot[sb] : Cmb::b_view_slot[sb];
+          if (slot >= 0) fm_put_planes<VEC>(vdst + ((int64_t)rr * NV + slot) * slab + pos, par[sb]);
+        }
+      }
+    };
+    fm_static_for<0, R>(product);
+  }
+}
+
 // Operands with at most this many blocks take the two-level stage
 // (NB * NB values per lane in registers).
 constexpr int FM_STAGE2_MAX_BLOCKS = 9;
@@ -438,6 +572,28 @@ constexpr FmStageLaunch fm_stage2_fn() {
     return nullptr;
 }
 
+template <class Cmb, int SIDE>
+hipError_t fm_launch_in_stage2_planes(const FmStageArgs& a, hipStream_t s) {
+  constexpr int VEC = fm_planes_vec<Cmb, SIDE>();
+  const uint64_t total = (uint64_t)a.rows2 * (uint64_t)a.cols2 / VEC;
+  uint64_t gx = (total + FM_STAGE_THREADS - 1) / FM_STAGE_THREADS;
+  uint64_t cap = 8192 / (uint64_t)(a.nodes > 0 ? a.nodes : 1);
+  if (cap < 4) cap = 4;
+  if (gx > cap) gx = cap;
+  if (gx < 1) gx = 1;
+  hipLaunchKernelGGL((fm_in_stage2_planes<Cmb, SIDE>), dim3((unsigned)gx, (unsigned)a.nodes, 1),
+                     dim3(FM_STAGE_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class Cmb, int SIDE>
+constexpr FmStageLaunch fm_stage2_planes_fn() {
+  if constexpr (fm_stage2_ok<Cmb, SIDE>())
+    return &fm_launch_in_stage2_planes<Cmb, SIDE>;
+  else
+    return nullptr;
+}
+
 #define FM_DEFINE_ALGORITHM(NAME, GNS, APA)                                                   \
   static hipError_t fm_##NAME##_s_d(const FmStageArgs& a, hipStream_t s) {                   \
     return fm_launch_in_stage<GNS::Combine, double, 0>(a, s);                                 \
@@ -467,6 +623,7 @@ constexpr FmStageLaunch fm_stage2_fn() {
         fm_stage2_fn<GNS::Combine, float, 0>(), fm_stage2_fn<GNS::Combine, float, 1>(),       \
         GNS::Combine::a_nviews, GNS::Combine::b_nviews, GNS::Combine::a_view_slot,            \
         GNS::Combine::b_view_slot, fm_w2_fn<GNS::Combine, double>(),                          \
-        fm_w2_fn<GNS::Combine, float>()};                                                    \
+        fm_w2_fn<GNS::Combine, float>(), fm_stage2_planes_fn<GNS::Combine, 0>(),             \
+        fm_stage2_planes_fn<GNS::Combine, 1>()};                                             \
     return &d;                                                                               \
   }

@@ -301,9 +301,12 @@ def set_schedule(dfs_levels: int = 0, workspace_limit: int = 0, split_remainder:
     check(LIB.fm_set_schedule(int(dfs_levels), int(workspace_limit), 1 if split_remainder else 0))
 
 
-def set_stage_fusion(on: bool = True):
-    """Two-level S/T stages where the shape allows them (bit-identical results)."""
-    LIB.fm_set_stage_fusion(1 if on else 0)
+def set_stage_fusion(on=True):
+    """Two-level S/T stages where the shape allows them (bit-identical
+    results).  True or 1: on, and with split_bf16 leaves the last S/T stage
+    writes the leaves' slice planes; 2: on, planes cut by a separate pass;
+    False or 0: one level at a time."""
+    LIB.fm_set_stage_fusion(2 if on == 2 else 1 if on else 0)
 
 
 def set_ozaki_slices(slices: int = 8):

@@ -94,7 +94,10 @@ fm_status fm_workspace_query(const char* alg, fm_dtype dtype, fm_mode mode, int 
  *   workspace_limit  bytes (0: none): depth-first levels are added from the
  *                    top until the plan fits (FM_ERR_WORKSPACE otherwise);
  *   split_remainder  1: whole-tile carve-out and the remainder launch;
- *   stage_fusion     1: two-level S/T/W stages where the shape allows;
+ *   stage_fusion     1: two-level S/T/W stages where the shape allows, and
+ *                    with split_bf16 leaves the last S/T stage writes the
+ *                    leaves' bf16 slice planes itself; 2: the same stages,
+ *                    the planes cut by a separate pass; 0: one level a time;
  *   ozaki_slices     6, 7 or 8 int8 digits (FM_MODE_OZAKI_INT8).
  * A NULL options pointer means the process defaults, which the fm_set_*
  * calls below change; fm_options_default reads them.  Results are
@@ -339,10 +342,14 @@ fm_status fm_set_schedule(int dfs_levels, size_t workspace_limit, int split_rema
  * Two-level S/T stages (default on): where two consecutive recursion levels
  * have no peeling between them, one launch forms the deeper level's S (T)
  * operands straight from the upper level's, without storing the middle
- * level.  Results are bit-identical either way; off trades speed for the
- * one-level schedule (plans built afterwards follow the setting).  No
- * counterpart in the reference, whose adds run one level at a time
- * (algorithms/strassen.hpp:9-252). */
+ * level.  With FM_MODE_SPLIT_BF16 leaves (FP32) whose rows are multiples of
+ * 256 and K of 32, the last such stage also writes the leaf operands as
+ * bf16 slice planes, which the leaf kernels read, instead of FP32 blocks
+ * that a preparation pass would cut into planes.  on: 1 (default), 2
+ * (two-level stages, planes by the separate pass), 0 (one level at a
+ * time).  Results are bit-identical under all three (plans built afterwards
+ * follow the setting).  No counterpart in the reference, whose adds run one
+ * level at a time (algorithms/strassen.hpp:9-252). */
 void fm_set_stage_fusion(int on);
 
 /* Drop cached plans and the library-owned workspace arenas.  The arenas (one

@@ -197,3 +197,46 @@ def test_split_bf16_huge_and_nonfinite_entries(fm, tile):
     got2, plain2 = run("split_bf16", Ad2, Bd), run("plain", Ad2, Bd)
     assert np.array_equal(np.isfinite(got2), np.isfinite(plain2))
     assert np.isnan(got2[40]).all()
+
+
+# Two-level S/T stages writing the leaves' slice planes themselves
+# (stage_fusion 1) against the same stages followed by the separate
+# preparation pass (stage_fusion 2): bit-identical.  The cases cover a lane
+# width of 4 (Strassen, 4 blocks a side), 2 (grey223's B side, 6 blocks) and
+# 1 (grey424, 8 blocks; smirnov333, 9 blocks), three levels, and alpha/beta.
+@pytest.mark.parametrize("alg,steps,shape,alpha,beta", [
+    ("strassen", 2, (1024, 1024, 1024), 1.0, 0.0),
+    ("strassen", 2, (2048, 512, 1024), 2.0, -1.0),
+    ("strassen", 3, (2048, 256, 2048), 1.0, 0.0),
+    ("grey223_11_50", 2, (1024, 512, 2304), 1.0, 0.0),
+    ("grey424_26_206", 2, (4096, 256, 4096), 1.0, 0.0),
+    ("smirnov333_23_139", 2, (2304, 288, 2304), 1.0, 0.0),
+])
+def test_split_planes_from_stages_bit_identical(fm, alg, steps, shape, alpha, beta):
+    m, k, n = shape
+    g = torch.Generator(device=DEV)
+    g.manual_seed(m + 7 * k + 13 * n + steps)
+    A = fm.col_major(torch.rand((m, k), generator=g, device=DEV) * 2 - 1)
+    B = fm.col_major(torch.rand((k, n), generator=g, device=DEV) * 2 - 1)
+    C0 = fm.col_major(torch.rand((m, n), generator=g, device=DEV) * 2 - 1)
+    # the fused plan prepares only the leaf operands that are sub-blocks of A or B
+    try:
+        fm.set_stage_fusion(1)
+        fused = fm.plan_info(alg, m, k, n, steps, dtype=torch.float32, mode="split_bf16")
+        fm.set_stage_fusion(2)
+        apart = fm.plan_info(alg, m, k, n, steps, dtype=torch.float32, mode="split_bf16")
+    finally:
+        fm.set_stage_fusion(1)
+    assert fused["prep_bytes"] < apart["prep_bytes"]
+    out = []
+    for fu in (1, 2):
+        C = C0.clone()
+        fm.fastmatmul(alg, A, B, C, steps, alpha=alpha, beta=beta, mode="split_bf16",
+                      opts=fm.options(stage_fusion=fu))
+        out.append(C)
+    torch.cuda.synchronize()
+    assert torch.equal(out[0], out[1])
+    # and accurate: FP32-level normwise error against the FP64 product
+    D = alpha * (A.double() @ B.double()) + beta * C0.double()
+    nw = float((out[0].double() - D).abs().max()) / k
+    assert nw <= 64 * 2.0 ** -24
