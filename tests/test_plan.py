@@ -37,13 +37,26 @@ def test_bench_workload_plan():
 def test_split_workload_plan():
     # Strassen, 2 levels, 16384^3 FP32 with split-bf16 leaves: 49 leaves of
     # 4096^3 on the kernel that loads B straight into registers (FmTileCfg
-    # 31, 128 x 256 tiles): 49 x 32 x 16 = 25088 tiles in one launch, the
-    # slice planes of 98 distinct operands (rows x K x 6 bytes each) on top
-    # of the plain FP32 plan's workspace
-    p = fastmm.plan_info("strassen", 16384, 16384, 16384, 2, dtype=torch.float32, mode="split_bf16")
+    # 31, 128 x 256 tiles): 49 x 32 x 16 = 25088 tiles in one launch.
+    # With the planes cut by a separate pass (stage_fusion 2): the slice
+    # planes of 98 distinct operands (rows x K x 6 bytes each) on top of the
+    # plain FP32 plan's workspace.  With the last S/T stage writing them
+    # (default): its 2 x 49 outputs and views take 6 bytes an element instead
+    # of 4, and only the 8 leaf operands that are sub-blocks of A or B get
+    # planes from the preparation pass
     q = fastmm.plan_info("strassen", 16384, 16384, 16384, 2, dtype=torch.float32)
+    p = fastmm.plan_info("strassen", 16384, 16384, 16384, 2, dtype=torch.float32, mode="split_bf16")
+    assert p["leaf"] == [{"cfg": 31, "products": 49, "tiles": 25088}]
+    assert p["workspace_bytes"] - q["workspace_bytes"] == 4096 * 4096 * (2 * 49 * 2 + 8 * 6)
+    assert p["prep_bytes"] == 8 * 4096 * 4096 * (4 + 6)
+    try:
+        fastmm.set_stage_fusion(2)
+        p = fastmm.plan_info("strassen", 16384, 16384, 16384, 2, dtype=torch.float32, mode="split_bf16")
+    finally:
+        fastmm.set_stage_fusion(1)
     assert p["leaf"] == [{"cfg": 31, "products": 49, "tiles": 25088}]
     assert p["workspace_bytes"] - q["workspace_bytes"] == 98 * 4096 * 4096 * 6
+    assert p["prep_bytes"] == 98 * 4096 * 4096 * (4 + 6)
 
 
 def test_one_level_stages_when_middle_level_peels():
