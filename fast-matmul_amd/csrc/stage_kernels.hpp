@@ -329,6 +329,10 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2_planes(FmStageA
   const double x = nv.x;
   for (uint32_t e = blockIdx.x * FM_STAGE_THREADS + threadIdx.x; e < total;
        e += gridDim.x * FM_STAGE_THREADS) {
+    // k fastest: adjacent lanes write adjacent bytes of a plane run, which
+    // the store path merges (with rows fastest instead, the A side's loads
+    // coalesce better but its stores do not: measured slower, S of
+    // grey424 16384 x 8192 3.2 -> 4.3 ms)
     const uint32_t kq = e % KQ;
     const uint32_t rest = e / KQ;
     const uint32_t r = rest % prow;
