@@ -240,3 +240,53 @@ def test_split_planes_from_stages_bit_identical(fm, alg, steps, shape, alpha, be
     D = alpha * (A.double() @ B.double()) + beta * C0.double()
     nw = float((out[0].double() - D).abs().max()) / k
     assert nw <= 64 * 2.0 ** -24
+
+
+def test_split_planes_plan_graph_replays(fm):
+    """A split-bf16 plan whose last S/T stage writes the slice planes, run
+    with a caller workspace and captured into a hipGraph: every replay (with
+    eager work between) equals the eager run, which equals the call with the
+    planes cut by the separate pass."""
+    m = k = n = 1024
+    g = torch.Generator(device=DEV)
+    g.manual_seed(41)
+    A = fm.col_major(torch.rand((m, k), generator=g, device=DEV) * 2 - 1)
+    B = fm.col_major(torch.rand((k, n), generator=g, device=DEV) * 2 - 1)
+    P = fm.Plan("strassen", m, k, n, 2, dtype=torch.float32, mode="split_bf16")
+    ws = torch.empty(max(1, P.workspace_bytes()), dtype=torch.uint8, device=DEV)
+    C = fm.empty(m, n, torch.float32, DEV)
+    eager = P.run(A, B, C, workspace=ws).clone()
+    apart = fm.fastmatmul("strassen", A, B, None, 2, mode="split_bf16",
+                          opts=fm.options(stage_fusion=2))
+    assert torch.equal(eager, apart)
+    s = torch.cuda.Stream(device=DEV)
+    graph = torch.cuda.CUDAGraph()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(graph, stream=s):
+        P.run(A, B, C, workspace=ws)
+    for i in range(3):
+        C.zero_()
+        junk = torch.randn(1000, device=DEV).sum()  # eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(C, eager), i
+        del junk
+
+
+def test_split_planes_scaled_bit_identical(fm):
+    """The diagonally scaled entry point takes the stage-written planes too:
+    bit-identical to the separate preparation pass."""
+    m = k = n = 1024
+    rng = np.random.default_rng(43)
+    A = (rng.uniform(-1, 1, (m, k)) * 10.0 ** rng.uniform(-3, 3, (m, 1))).astype(np.float32)
+    B = (rng.uniform(-1, 1, (k, n)) * 10.0 ** rng.uniform(-3, 3, (1, n))).astype(np.float32)
+    Ad, Bd = _dev(A, fm), _dev(B, fm)
+    out = []
+    try:
+        for fu in (1, 2):
+            fm.set_stage_fusion(fu)
+            out.append(fm.fastmatmul_scaled("strassen", Ad, Bd, None, 2, mode="split_bf16"))
+    finally:
+        fm.set_stage_fusion(1)
+    torch.cuda.synchronize()
+    assert torch.equal(out[0], out[1])
