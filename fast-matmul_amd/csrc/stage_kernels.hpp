@@ -312,6 +312,14 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2_planes(FmStageA
   constexpr int NV = SIDE == 0 ? Cmb::a_nviews : Cmb::b_nviews;
   constexpr int VEC = fm_planes_vec<Cmb, SIDE>();
   constexpr int KQ = 32 / VEC;  // lanes per 32-deep run of a plane row
+  constexpr int RT = FM_STAGE_THREADS / KQ;  // plane rows of one work group's lanes
+  // A side at lane widths 1 and 2: the sources go through LDS (below), where
+  // their tile leaves room for two work groups per CU (measured, S stage:
+  // grey424, 8 blocks, 74 KB: 3.21 -> 2.52 ms at 16384 x 8192; smirnov333,
+  // 9 blocks, 93 KB, one group per CU: 1.20 -> 1.67 ms, so not staged)
+  constexpr bool STAGED =
+      SIDE == 0 && VEC <= 2 && NB * NB * 32 * (RT + 1) * (int)sizeof(T) <= 80 * 1024;
+  __shared__ T stg[STAGED ? NB * NB * 32 : 1][RT + 1];  // [source x column][row]
   const int q = blockIdx.y;
   const FmNodeView nv = fm_node_view(a, q, R * NOUT, R * NV);
   const FmOp op = nv.op;
@@ -340,17 +348,50 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2_planes(FmStageA
     const uint32_t k0 = kc * 32 + kq * VEC;
     const uint32_t i = SIDE == 0 ? r : k0, j = SIDE == 0 ? k0 : r;
     T v[NB][NB][VEC];  // v[block][sub-block][element]
-    const fm_gptr<const T> lsrc = src + i + (int64_t)j * ld;
+    if constexpr (STAGED) {
+      // The work group's lanes cover RT consecutive plane rows (A rows
+      // r0 ..) of one 32-deep run (columns kc * 32 ..): every lane takes the
+      // same trip count (the total is a multiple of 256, plane rows a
+      // multiple of RT).  Each source's RT x 32 tile is loaded rows-first
+      // (RT consecutive rows of a column per load) into LDS, then each lane
+      // reads its k-first elements.  The k-first loads straight from HBM
+      // fetched 32 columns' lines for 2 rows per wave instruction.
+      const uint32_t r0 = ((e - threadIdx.x) / KQ) % prow;
+      const uint32_t sr = threadIdx.x % RT, sc = threadIdx.x / RT;
+      const fm_gptr<const T> ssrc = src + (r0 + sr) + (int64_t)(kc * 32 + sc) * ld;
 #pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const fm_gptr<const T> bsrc = lsrc + (int64_t)(b / BC) * brows + (int64_t)(b % BC) * bcols * ld;
+      for (int b = 0; b < NB; ++b) {
+        const fm_gptr<const T> bsrc = ssrc + (int64_t)(b / BC) * brows + (int64_t)(b % BC) * bcols * ld;
 #pragma unroll
-      for (int sb = 0; sb < NB; ++sb) {
-        const fm_gptr<const T> p = bsrc + (int64_t)(sb / BC) * brows2 + (int64_t)(sb % BC) * bcols2 * ld;
+        for (int sb = 0; sb < NB; ++sb) {
+          const fm_gptr<const T> p = bsrc + (int64_t)(sb / BC) * brows2 + (int64_t)(sb % BC) * bcols2 * ld;
 #pragma unroll
-        for (int l = 0; l < VEC; ++l) v[b][sb][l] = p[l * estep];
+          for (int pp = 0; pp < VEC; ++pp) stg[(b * NB + sb) * 32 + sc + pp * KQ][sr] = p[(int64_t)pp * KQ * ld];
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-      __builtin_amdgcn_sched_barrier(0);
+      __syncthreads();
+      const uint32_t rl = threadIdx.x / KQ;  // r - r0
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int sb = 0; sb < NB; ++sb)
+#pragma unroll
+          for (int l = 0; l < VEC; ++l) v[b][sb][l] = stg[(b * NB + sb) * 32 + kq * VEC + l][rl];
+      __syncthreads();  // the next trip overwrites the tile
+    } else {
+      const fm_gptr<const T> lsrc = src + i + (int64_t)j * ld;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        const fm_gptr<const T> bsrc = lsrc + (int64_t)(b / BC) * brows + (int64_t)(b % BC) * bcols * ld;
+#pragma unroll
+        for (int sb = 0; sb < NB; ++sb) {
+          const fm_gptr<const T> p = bsrc + (int64_t)(sb / BC) * brows2 + (int64_t)(sb % BC) * bcols2 * ld;
+#pragma unroll
+          for (int l = 0; l < VEC; ++l) v[b][sb][l] = p[l * estep];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
     }
     // the lane's run within one slab's planes
     const int64_t pos = ((int64_t)kc * prow + r) * 192 + kq * (VEC * 2);
