@@ -56,6 +56,9 @@ struct FmOp {
 // Two-level W stage: ops[q] are the level-l C operands (blocks rows x cols,
 // sub-blocks rows2 x cols2); the level-(l+2) products of node q are at
 // ws + out_off + ((q * R + r) * R + r2) * out_slab.
+// Two-level S/T stage writing split-bf16 slice planes (FP32 data): the same
+// slab numbering, with out_off, view_off and out_slab in bytes (one slab:
+// the planes of a rows2 x cols2 operand) and out_ld unused.
 //
 // Grouped stage launch (the nodes of several problems, whose blocks differ
 // in shape, in one launch): `node` holds each node's operand, shape, output
