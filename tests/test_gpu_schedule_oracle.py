@@ -101,3 +101,30 @@ def test_split_bf16_tiles_vs_oracle(fm, oracle_mod, golden, cid, tile):
     C = fm.fastmatmul(alg, _dev(A, fm), _dev(B, fm), _dev(C0, fm), steps, x, alpha, beta,
                       mode="split_bf16", opts=dict(gemm_tile=tile))
     assert _normwise(C.cpu().numpy(), Co, A, B) <= tol
+
+
+# Split-bf16 leaves whose operands' slice planes the last two-level S/T
+# stage writes itself (the default where the leaves' rows are multiples of
+# 256 and their K of 32; the golden cases are too small for that path):
+# against the oracle's FP32 run on the same seeded inputs, with e_ref the
+# oracle's own normwise error against the FP64 product.
+@pytest.mark.parametrize("alg,steps,shape", [("strassen", 2, (1024, 1024, 1024)),
+                                             ("grey424_26_206", 2, (4096, 256, 4096)),
+                                             ("smirnov333_23_139", 2, (2304, 288, 2304))])
+def test_split_planes_from_stages_vs_oracle(fm, oracle_mod, alg, steps, shape):
+    m, k, n = shape
+    A, B, _ = make_inputs(m, k, n, 51, "u11", np.float32)
+    x = x_for(alg, steps)
+    Co = oracle_mod.fastmatmul(alg, A, B, None, steps, 1.0, 0.0, x)
+    D = A.astype(np.float64) @ B.astype(np.float64)
+    e_ref = _normwise(Co, D, A, B)
+    try:
+        fm.set_stage_fusion(1)
+        fused = fm.plan_info(alg, m, k, n, steps, dtype=torch.float32, mode="split_bf16")
+        fm.set_stage_fusion(2)
+        apart = fm.plan_info(alg, m, k, n, steps, dtype=torch.float32, mode="split_bf16")
+    finally:
+        fm.set_stage_fusion(1)
+    assert fused["prep_bytes"] < apart["prep_bytes"]  # the planes come from the stage
+    C = fm.fastmatmul(alg, _dev(A, fm), _dev(B, fm), None, steps, x, mode="split_bf16")
+    assert _normwise(C.cpu().numpy(), Co, A, B) <= 2 * e_ref + 1e-6
