@@ -631,9 +631,19 @@ hipError_t fm_launch_in_stage2_planes(const FmStageArgs& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Plane-writing stages for tables up to NB^2 R = 2048 (every table up to
+// <3,3,3;23> and <4,2,4;26>): beyond that (the R = 40 and R = 29 families,
+// classical333) the fully unrolled kernel triples the file's compile time
+// (<6,3,3;40>: 57 -> 182 s) and those tables keep the separate preparation.
+template <class Cmb, int SIDE>
+constexpr int fm_planes_weight() {
+  constexpr int NB = SIDE == 0 ? Cmb::m * Cmb::k : Cmb::k * Cmb::n;
+  return NB * NB * Cmb::R;
+}
+
 template <class Cmb, int SIDE>
 constexpr FmStageLaunch fm_stage2_planes_fn() {
-  if constexpr (fm_stage2_ok<Cmb, SIDE>())
+  if constexpr (fm_stage2_ok<Cmb, SIDE>() && fm_planes_weight<Cmb, SIDE>() <= 2048)
     return &fm_launch_in_stage2_planes<Cmb, SIDE>;
   else
     return nullptr;
