@@ -203,3 +203,28 @@ def test_benchmark_output_format_reads_reference_data():
     assert bm.parse(line) == {"FAST424_26_206_2": [(8192, 8192, 8192, 2, 14.2),
                                                     (4096, 4096, 4096, 2, 2.1)]}
     assert bm.enum_name("strassen") == "STRASSEN" and bm.enum_name("MKL") == "MKL"
+
+
+def test_split_planes_from_stages_only_in_breadth_first_plans():
+    # Strassen, 3 levels, 8192^3 FP32 split-bf16: breadth-first, the
+    # two-level S/T stage of levels 1-3 writes the leaves' planes; the leaf
+    # operands that are sub-blocks of a level-1 operand (4 per side for each
+    # of the 7 level-1 nodes) take the preparation pass: 56 x 1024^2 x
+    # (4 + 6) bytes.  Depth-first children cut all their planes in the
+    # preparation pass.
+    args = ("strassen", 8192, 8192, 8192, 3)
+    p = fastmm.plan_info(*args, dtype=torch.float32, mode="split_bf16")
+    assert p["prep_bytes"] == 56 * 1024 * 1024 * (4 + 6)
+    try:
+        fastmm.set_stage_fusion(2)
+        q = fastmm.plan_info(*args, dtype=torch.float32, mode="split_bf16")
+        fastmm.set_stage_fusion(1)
+        fastmm.set_schedule(1, 0, True)
+        d1 = fastmm.plan_info(*args, dtype=torch.float32, mode="split_bf16")
+        fastmm.set_stage_fusion(2)
+        d2 = fastmm.plan_info(*args, dtype=torch.float32, mode="split_bf16")
+    finally:
+        fastmm.set_stage_fusion(1)
+        fastmm.set_schedule(0, 0, True)
+    assert p["prep_bytes"] < q["prep_bytes"]
+    assert d1["dfs_levels"] == 1 and d1["prep_bytes"] == d2["prep_bytes"]
