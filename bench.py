@@ -392,8 +392,8 @@ def main():
                          "known-size reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the fast multiply on a single MI355X "
-              "(multi-GPU sharding is a design note, not built); run with --gpus 1",
+        print("bench.py measures one GPU: the fast multiply runs on a single MI355X "
+              "(multi-GPU sharding is designed, not built); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 
