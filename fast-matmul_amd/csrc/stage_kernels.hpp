@@ -457,9 +457,12 @@ constexpr int FM_STAGE2_MAX_BLOCKS = 9;
 // Same operations in the same order as the two one-level W stages.
 template <class Cmb, typename T>
 constexpr int fm_w2_rows() {
-  // rows per work group: LDS holds R * m*n * P values, at most 64 KB
+  // rows per work group: LDS holds R * m*n * P values, at most 32 KB for
+  // 64 or 128 rows (small tables: longer runs of each product read), 64 KB
+  // for 32 rows and fewer
   constexpr int per = Cmb::R * Cmb::m * Cmb::n * (int)sizeof(T);
-  return per * 32 <= 65536 ? 32 : per * 16 <= 65536 ? 16 : per * 8 <= 65536 ? 8 : 0;
+  return per * 128 <= 32768 ? 128 : per * 64 <= 32768 ? 64 : per * 32 <= 65536 ? 32
+       : per * 16 <= 65536 ? 16 : per * 8 <= 65536 ? 8 : 0;
 }
 
 template <class Cmb, typename T, bool BETA>
