@@ -54,6 +54,8 @@ def test_noalloc_two_calls_second_smaller(fm):
 
 @pytest.mark.parametrize("mode,dtype,alg,shape", [
     ("split_bf16", np.float32, "strassen", (1024, 768, 512)),
+    # leaves of 256^3: the last S/T stage writes the slice planes into the array
+    ("split_bf16", np.float32, "strassen", (1024, 1024, 1024)),
     ("ozaki_int8", np.float64, "smirnov333_23_139", (486, 486, 486)),
     ("plain", np.float32, "grey234_20_144", (130, 290, 250)),   # peeling at both levels
 ])
