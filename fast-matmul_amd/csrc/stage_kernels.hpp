@@ -313,12 +313,12 @@ __global__ void __launch_bounds__(FM_STAGE_THREADS) fm_in_stage2_planes(FmStageA
   constexpr int VEC = fm_planes_vec<Cmb, SIDE>();
   constexpr int KQ = 32 / VEC;  // lanes per 32-deep run of a plane row
   constexpr int RT = FM_STAGE_THREADS / KQ;  // plane rows of one work group's lanes
-  // A side at lane widths 1 and 2: the sources go through LDS (below), where
-  // their tile leaves room for two work groups per CU (measured, S stage:
-  // grey424, 8 blocks, 74 KB: 3.21 -> 2.52 ms at 16384 x 8192; smirnov333,
-  // 9 blocks, 93 KB, one group per CU: 1.20 -> 1.67 ms, so not staged)
-  constexpr bool STAGED =
-      SIDE == 0 && VEC <= 2 && NB * NB * 32 * (RT + 1) * (int)sizeof(T) <= 80 * 1024;
+  // A side: the sources go through LDS (below) where their tile leaves room
+  // for two work groups per CU (measured, S stage: grey424, 8 blocks, width
+  // 1, 74 KB: 3.21 -> 2.52 ms at 16384 x 8192; Strassen, width 4, 68 KB:
+  // 1.32 -> 1.19 ms at 16384^3; smirnov333, 9 blocks, 93 KB, one group per
+  // CU: 1.20 -> 1.67 ms, so not staged)
+  constexpr bool STAGED = SIDE == 0 && NB * NB * 32 * (RT + 1) * (int)sizeof(T) <= 80 * 1024;
   __shared__ T stg[STAGED ? NB * NB * 32 : 1][RT + 1];  // [source x column][row]
   const int q = blockIdx.y;
   const FmNodeView nv = fm_node_view(a, q, R * NOUT, R * NV);
