@@ -544,6 +544,14 @@ def main():
     t_roof = F_leaf / (PEAK_FP64_TF * 1e12) + b_add / (PEAK_HBM_GBS * 1e9)
     out["multiply_roofline"] = {"t_roof_ms": t_roof * 1e3, "t_measured_ms": t * 1e3,
                                 "frac": t_roof / t, "leaf_flops": F_leaf, "stage_bytes": b_add}
+    # the same bound with each stage at the measured rate of its own access
+    # pattern (fm_probe_stream) instead of the HBM nameplate
+    pat = [stages[k] for k in ("S", "T", "W") if stages[k].get("algorithmic_GB")]
+    if pat and all(isinstance(s.get("pattern_GBps"), float) for s in pat):
+        t_pat = F_leaf / (PEAK_FP64_TF * 1e12) + sum(
+            s["algorithmic_GB"] / s["pattern_GBps"] for s in pat)
+        out["multiply_roofline"].update({"t_roof_pattern_ms": t_pat * 1e3,
+                                         "frac_pattern": t_pat / t})
     print(json.dumps(out))
 
 
